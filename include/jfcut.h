@@ -1,0 +1,248 @@
+/*
+ * jfcut.h -- C-ABI of libjfcut, the MI355X (gfx950) grid max-flow / min-cut
+ * engine that replaces JF-Cut's graph-cut hot path.
+ *
+ * The reference has no library boundary: its operator surface is implicit in
+ * QGCWidget (src/QVisualizer/Graph Cut/QGCWidget.cpp, "QGCW" below).  Each
+ * entry point names the reference code it stands in for.
+ *
+ *   graph  = one or more grid instances (2D 4- or 8-connected, 3D
+ *            6-connected), int32 structure-of-arrays capacity planes in HBM.
+ *   solve  = block-wise push-relabel on LDS tiles with exact global relabel,
+ *            run on the device until no node with excess can reach a node
+ *            with deficit; then the canonical cut (nodes reachable from a node
+ *            with excess in the residual graph) and the int64 max-flow value.
+ *
+ * Conventions
+ *   - Node v = x + nx*(y + ny*z).  Direction d of a connectivity:
+ *       2D: 0:+x 1:-x 2:+y 3:-y 4:(+x,+y) 5:(-x,-y) 6:(-x,+y) 7:(+x,-y)
+ *       3D: 0:+x 1:-x 2:+y 3:-y 4:+z 5:-z
+ *     The reverse of direction d is d^1.  Neighbour plane d holds, per node
+ *     v, the capacity of the arc v -> v + off(d); arcs leaving the grid are
+ *     ignored (stored as 0).
+ *   - Terminal capacities: 0 <= cap <= JFC_MAX_TERMINAL_CAP (2^30, the hard
+ *     seed capacity).  Neighbour capacities: 0 <= cap <= JFC_MAX_NEIGHBOR_CAP.
+ *     Larger values are rejected with JFC_ERR_OVERFLOW, so every intermediate
+ *     value of the solve fits int32 and the flow is exact.
+ *   - Pointers named *_device (or passed with on_device = 1) are HIP device
+ *     pointers; all others are host pointers.  Inputs are copied; the caller
+ *     keeps ownership.
+ *   - Every function returns JFC_OK (0) or a negative JFC_ERR_* code, except
+ *     jfc_solve, which may also return JFC_NOT_CONVERGED (1).  The message of
+ *     the last error of the calling thread is returned by jfc_last_error().
+ *   - A graph may be used by one host thread at a time; distinct graphs are
+ *     independent.  Work is issued on the stream given in jfc_options (NULL =
+ *     the default stream) and every call returns after that work completed.
+ */
+#ifndef JFCUT_H
+#define JFCUT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JFC_OK 0
+#define JFC_NOT_CONVERGED 1
+#define JFC_ERR_INVALID (-1)
+#define JFC_ERR_DEVICE (-2)
+#define JFC_ERR_OOM (-3)
+#define JFC_ERR_STATE (-4)
+#define JFC_ERR_OVERFLOW (-5)
+#define JFC_ERR_NO_SEEDS (-6)
+
+#define JFC_MAX_TERMINAL_CAP (1 << 30)
+#define JFC_MAX_NEIGHBOR_CAP (1 << 26)
+
+typedef struct jfc_graph jfc_graph;
+
+/* Solver options (the reference's per-tab UI values read in
+ * QGCW:2364-2374 and QGCSetting.cpp:19-36, gathered in one struct). */
+typedef struct {
+  int max_sweeps;         /* totalIterations: cap on push-relabel sweeps (each sweep
+                             visits every colour class of active tiles once) */
+  int check_every;        /* countIterations: sweeps between device->host
+                             convergence checks */
+  double relabel_work;    /* run an exact global relabel once the tiles visited
+                             since the last one exceed this many times the tile
+                             count (0 = only at start and on certificate failure) */
+  int max_local_iters;    /* cap on LDS discharge iterations per tile visit */
+  int tile_shape;         /* JFC_TILE_* (0 = default for the dimensionality) */
+  int profile;            /* 1 = time every push-relabel launch with HIP events */
+  void *stream;           /* hipStream_t to issue work on (NULL = default stream) */
+} jfc_options;
+
+#define JFC_TILE_DEFAULT 0
+#define JFC_TILE_32x8 1
+#define JFC_TILE_64x4 2
+#define JFC_TILE_16x16 3
+#define JFC_TILE_8x8x4 4
+#define JFC_TILE_32x4x2 5
+
+typedef struct {
+  int64_t flow;             /* max-flow of instance 0 (all instances: jfc_get_flows) */
+  int converged;            /* 1 if every instance reached the certificate */
+  int sweeps;               /* push-relabel sweeps run */
+  int global_relabels;      /* exact global relabels run */
+  int relabel_passes;       /* tile passes inside the global relabels */
+  int cut_passes;           /* tile passes of the canonical-cut search */
+  int certificate_failures; /* cut searches that still found an augmenting path */
+  int64_t push_tiles;       /* tile visits of the push-relabel kernel */
+  int64_t push_launches;    /* push-relabel kernel launches */
+  int64_t relabel_tiles;    /* tile visits of the relabel kernel */
+  int64_t cut_tiles;        /* tile visits of the cut kernel */
+  int64_t push_bytes;       /* algorithmic HBM bytes of the push-relabel visits */
+  double ms_total;          /* device time of the solve (HIP events) */
+  double ms_push;           /* summed push-relabel launch time (profile = 1) */
+  double ms_relabel;        /* global relabel phases */
+  double ms_cut;            /* canonical cut + certificate */
+} jfc_stats;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Fill *opt with the defaults. */
+void jfc_default_options(jfc_options *opt);
+
+/* Create a graph of one instance.  Replaces the buffer set-up of
+ * QGCW:815-887 (initConfigurations) and QGCW:955-958 (devExcessFlow,
+ * devHeight, devCapacity1/2).  dims is 2 or 3; connectivity 4 or 8 (2D) or 6
+ * (3D); nz must be 1 for 2D. */
+int jfc_create(jfc_graph **g, int dims, int nx, int ny, int nz, int connectivity);
+
+/* Create a graph of `count` independent instances solved together (no
+ * reference counterpart: the reference solves one image at a time).
+ * shapes holds count rows of (nx, ny, nz, connectivity); nz == 1 means 2D. */
+int jfc_create_batch(jfc_graph **g, int count, const int32_t *shapes);
+
+/* Release every resource of the graph (the widget destructor's buffer
+ * release, QGCWidget.h:103-131).  NULL is ignored. */
+void jfc_destroy(jfc_graph *g);
+
+int jfc_num_instances(const jfc_graph *g);
+
+/* Thread-local message of the last error (QError / cl::Error text,
+ * infrastructures/QError.h:37-74, QGCW:2533-2557). */
+const char *jfc_last_error(void);
+
+/* ---- inputs (QGCW:2631-2722 slotLoadGraphClicked, 2560-2606 slotModelClicked) */
+
+/* Source / sink capacities of instance `inst` (N values each). */
+int jfc_set_terminal_caps(jfc_graph *g, int inst, const int32_t *cap_source,
+                          const int32_t *cap_sink, int on_device);
+
+/* Terminal capacities in the reference's folded form: one signed int32 per
+ * node, ef > 0 = source capacity, ef < 0 = sink capacity (the .ef file /
+ * devExcessFlow, QGCW:2631-2722). */
+int jfc_set_excess(jfc_graph *g, int inst, const int32_t *ef, int on_device);
+
+/* Neighbour capacities of direction `arc` (N values). */
+int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int32_t *cap,
+                          int on_device);
+
+/* Hard seeds: fg[v] != 0 -> (source, sink) = (2^30, 0); bg[v] != 0 ->
+ * (0, 2^30); applied on top of the terminal capacities already set (fg wins
+ * where both are set).  Either pointer may be NULL.  Stands in for the
+ * cl_cut foreground/background bytes and the seed rule of
+ * graphcut_compute_excessflow.cl:71-74. */
+int jfc_set_seeds(jfc_graph *g, int inst, const uint8_t *fg, const uint8_t *bg,
+                  int on_device);
+
+/* ---- solve (QGCW:2364-2558 slotCutClicked) --------------------------------- */
+
+/* Run every instance to the max-flow certificate or opt->max_sweeps.
+ * opt and stats may be NULL.  Returns JFC_OK, JFC_NOT_CONVERGED or an error. */
+int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats);
+
+/* ---- outputs --------------------------------------------------------------- */
+
+/* Max-flow value of one instance = min-cut energy (QGCW:2452-2520). */
+int jfc_get_flow(jfc_graph *g, int inst, int64_t *flow);
+
+/* Max-flow values of all instances (count values). */
+int jfc_get_flows(jfc_graph *g, int64_t *flows);
+
+/* Canonical labels: 1 = source side (reachable from the source in the
+ * residual graph), 0 = sink side.  Equals cl_cut.object of the reference at
+ * convergence (graphcut_compute_cut.cl:19-140). */
+int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_device);
+
+/* Cut energy of the current labels on the input capacities, computed on the
+ * device (QGCW:2452-2520); equals the flow once solved. */
+int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy);
+
+/* Read back the input capacities of an instance, as set or as built by
+ * jfc_model_image (the graph save of QGCW:2724-2736, slotSaveGraphClicked). */
+int jfc_get_terminal_caps(jfc_graph *g, int inst, int32_t *cap_source,
+                          int32_t *cap_sink, int on_device);
+int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *cap,
+                          int on_device);
+
+/* ---- backward (no reference counterpart) --------------------------------- */
+
+/* Exact subgradient of the min-cut energy E = sum_T cs + sum_S ct +
+ * sum_{u in S, v in T} c(u->v) for one instance, as fp32:
+ *   d_cap_source[v] = 1[v in T], d_cap_sink[v] = 1[v in S],
+ *   d_neighbor[d*N + v] = 1[v in S and v+off(d) in T] (K planes).
+ * Any output may be NULL.  Call after jfc_solve. */
+int jfc_backward(jfc_graph *g, int inst, float *d_cap_source, float *d_cap_sink,
+                 float *d_neighbor, int on_device);
+
+/* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
+
+typedef struct {
+  int k_fg, k_bg;        /* clusters per side (reference spinBoxKMeans, <= 16) */
+  int kmeans_iters;      /* max Lloyd iterations (spinBoxKMeansIterations) */
+  int64_t unary_scale;   /* unary = round(unary_scale * d2 / range^2) */
+  int64_t range;         /* intensity range in the unary (255 for 8-bit) */
+  double lambda_xy;      /* pairwise weight in x/y (and 2D diagonals / sqrt 2) */
+  double lambda_z;       /* pairwise weight along z */
+  int z_separate;        /* 1: z arcs use their own mean squared difference */
+} jfc_model_params;
+
+typedef struct {
+  int iters_fg, iters_bg;      /* Lloyd iterations run per side */
+  int64_t centroids_fg[16 * 3];/* fixed-point centroids (value * 256) */
+  int64_t centroids_bg[16 * 3];
+  int64_t seeds_fg, seeds_bg;  /* seeded node counts */
+  int64_t pair_sum[2];         /* sum of squared neighbour differences per group */
+  int64_t pair_count[2];       /* edges per group (0 = xy, 1 = z if separate) */
+} jfc_model_stats;
+
+/* Build the whole graph of instance `inst` from an image and seed map on the
+ * device: integer k-means per side on the seeded pixels, unary terms from the
+ * distance to the nearest centroid of the opposite model, hard seeds, and
+ * pairwise weights round(lambda * exp(-D / (2 s2))) with s2 the mean squared
+ * neighbour difference.  image: N * channels values of 1 or 2 bytes
+ * (channels 1 or 3); seeds: N bytes, 1 = foreground, 2 = background. */
+int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
+                    int bytes_per_channel, const uint8_t *seeds, int on_device,
+                    const jfc_model_params *params, jfc_model_stats *stats);
+
+/* ---- synthetic workloads (benchmark inputs; no reference counterpart) ---- */
+
+/* 2D RGB image: fg ellipse at the centre (semi-axes 150x100 scaled by
+ * min(W,H)/512, mean colour (200,80,60)) on background (40,90,160), nblobs
+ * Gaussian colour blobs (sigma 20-60 px, amplitude +-40 per channel),
+ * Gaussian noise sigma `noise`, clipped to 0..255; 5-px scribbles: 2 inside
+ * the ellipse (seed 1) and 2 in the bands above / below it (seed 2). */
+int jfc_synth_image(uint8_t *rgb, uint8_t *seeds, int W, int H, int nblobs,
+                    double noise, uint64_t seed);
+
+/* Video volume W x H x T RGB: the ellipse of jfc_synth_image translates
+ * 4 px/frame along a random direction and rotates 1 degree/frame; noise
+ * sigma `noise` per frame; scribbles on the first and last frame only. */
+int jfc_synth_video(uint8_t *rgb, uint8_t *seeds, int W, int H, int T, int nblobs,
+                    double noise, uint64_t seed);
+
+/* uint16 volume: `nell` random ellipsoids (radii 20-120, intensity
+ * 1200-3000) over 400, 3D Gaussian smoothing (sigma 2), noise sigma 80;
+ * 3 seven-voxel-wide strokes in the largest ellipsoid (seed 1) and 3 in the
+ * background (seed 2). */
+int jfc_synth_volume(uint16_t *vol, uint8_t *seeds, int nx, int ny, int nz,
+                     int nell, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

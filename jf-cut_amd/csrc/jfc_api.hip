@@ -1,0 +1,825 @@
+// jfc_api.hip -- host driver and C-ABI of libjfcut (include/jfcut.h).
+//
+// The solve driver replaces QGCWidget::slotCutClicked
+// (src/QVisualizer/Graph Cut/QGCWidget.cpp:2364-2558).  Differences by design:
+//   - the loop stays on the device: tile lists are rebuilt by ballot
+//     compaction kernels and consumed by persistent workgroups, and the host
+//     reads one counter every `check_every` sweeps instead of several
+//     blocking readbacks per iteration;
+//   - convergence is decided by the active-tile count, which is exact because
+//     heights stay a valid distance labelling (global relabel is an exact
+//     BFS, colour classes never touch each other's nodes); the canonical-cut
+//     search then doubles as the certificate (QGCWidget.cpp:3237-3273);
+//   - the max-flow value is an int64 reduction of the absorbed deficit
+//     (reduce_flow in the reference accumulates int32, reduce.cl:17-42).
+#include "jfcut.h"
+#include "jfcut_debug.h"
+#include "jfc_internal.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace jfc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define JFC_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(JFC_ERR_DEVICE, std::string(#call) + ": " +               \
+                                      hipGetErrorString(e_));                \
+  } while (0)
+
+template <class T>
+int dalloc(T **p, size_t n, const char *what) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(JFC_ERR_OOM, std::string("device allocation failed: ") + what +
+                                 " (" + std::to_string(n * sizeof(T)) + " bytes)");
+  }
+  return JFC_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+int default_shape(int dim) { return dim == 3 ? SH_8x8x4 : SH_32x8; }
+
+int shape_from_option(int opt, int dim) {
+  switch (opt) {
+    case JFC_TILE_32x8: return dim == 2 ? SH_32x8 : 0;
+    case JFC_TILE_64x4: return dim == 2 ? SH_64x4 : 0;
+    case JFC_TILE_16x16: return dim == 2 ? SH_16x16 : 0;
+    case JFC_TILE_8x8x4: return dim == 3 ? SH_8x8x4 : 0;
+    case JFC_TILE_32x4x2: return dim == 3 ? SH_32x4x2 : 0;
+    default: return default_shape(dim);
+  }
+}
+
+// algorithmic HBM bytes per node of one push-relabel visit: read and write
+// e, h and K residual planes (int32)
+long long push_bytes_per_node(int K) { return 2LL * 4 * (2 + K); }
+
+}  // namespace
+
+struct jfc_graph {
+  int dim = 2;
+  int Kg = 4;  // planes stored (max K over instances)
+  int shape = 0;
+  long long Ntot = 0;
+  std::vector<InstDesc> inst;
+  std::vector<int> caps_set;  // per instance: bit 0 terminals, bit 1 + d neighbour planes
+  // tile tables for the current shape
+  int ntiles = 0, ncolors = 2, max_tile_dim = 1, nbins = 2;
+  std::vector<int> color_off;
+  // device planes
+  int *cs = nullptr, *ct = nullptr, *c0 = nullptr, *cap = nullptr;
+  int *e = nullptr, *h = nullptr;
+  unsigned char *lab = nullptr;
+  // device tables / work space
+  InstDesc *d_inst = nullptr;
+  int *d_tile_inst = nullptr, *d_color_tiles = nullptr;
+  int *d_list[2] = {nullptr, nullptr};
+  int *d_ctr = nullptr;
+  unsigned char *d_flag = nullptr, *d_dirty = nullptr, *d_seed = nullptr;
+  int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
+      *d_order = nullptr;
+  long long *d_red = nullptr, *d_total = nullptr;
+  int *d_sweep = nullptr;
+  int sweep_cap = 0;
+  int *d_flagword = nullptr;  // [0] certificate, [1] validation
+  int slot = 0;
+  bool solved = false;
+  std::vector<long long> flows;
+  std::vector<hipEvent_t> ev_pool;
+  int nwg = 2048;
+
+  ~jfc_graph() { release(); }
+
+  void release_tiles() {
+    dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
+    dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
+    dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst);
+  }
+  void release() {
+    release_tiles();
+    dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab);
+    dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
+    for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
+    ev_pool.clear();
+  }
+
+  Tiles tiles() const { return Tiles{d_inst, d_tile_inst, ntiles}; }
+  Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot}; }
+  long long inst_nodes(int i) const {
+    return (long long)inst[i].nx * inst[i].ny * inst[i].nz;
+  }
+
+  // Build tile tables, colour classes and instance descriptors for `sh`.
+  int set_shape(int sh) {
+    if (sh == shape && d_tile_inst) return JFC_OK;
+    release_tiles();
+    shape = sh;
+    int tw, th, td;
+    shape_dims(sh, &tw, &th, &td);
+    bool all4 = true;
+    int off = 0;
+    max_tile_dim = 1;
+    int maxsum = 0;
+    for (auto &I : inst) {
+      I.tx = (I.nx + tw - 1) / tw;
+      I.ty = (I.ny + th - 1) / th;
+      I.tz = (I.nz + td - 1) / td;
+      I.tile_off = off;
+      I.ntiles = I.tx * I.ty * I.tz;
+      off += I.ntiles;
+      if (I.K != 4) all4 = false;
+      max_tile_dim = std::max(max_tile_dim, std::max(I.tx, std::max(I.ty, I.tz)));
+      maxsum = std::max(maxsum, I.tx + I.ty + I.tz);
+    }
+    ntiles = off;
+    nbins = maxsum + 2;
+    // colour classes: tiles of one class share no node or halo cell.
+    // 2D 4C and 3D 6C: checkerboard; 2D 8C: 2x2 pattern.
+    ncolors = (dim == 2 && !all4) ? 4 : 2;
+    std::vector<int> tile_inst(ntiles);
+    std::vector<std::vector<int>> cls(ncolors);
+    for (size_t i = 0; i < inst.size(); ++i) {
+      const InstDesc &I = inst[i];
+      for (int z = 0; z < I.tz; ++z)
+        for (int y = 0; y < I.ty; ++y)
+          for (int x = 0; x < I.tx; ++x) {
+            const int t = I.tile_off + x + I.tx * (y + I.ty * z);
+            tile_inst[t] = (int)i;
+            const int c = ncolors == 4 ? ((x & 1) | ((y & 1) << 1)) : ((x + y + z) & 1);
+            cls[c].push_back(t);
+          }
+    }
+    std::vector<int> flat;
+    color_off.assign(ncolors + 1, 0);
+    for (int c = 0; c < ncolors; ++c) {
+      color_off[c] = (int)flat.size();
+      flat.insert(flat.end(), cls[c].begin(), cls[c].end());
+    }
+    color_off[ncolors] = (int)flat.size();
+    int rc;
+    if ((rc = dalloc(&d_inst, inst.size(), "instances")) ||
+        (rc = dalloc(&d_tile_inst, ntiles, "tile table")) ||
+        (rc = dalloc(&d_color_tiles, ntiles, "colour classes")) ||
+        (rc = dalloc(&d_list[0], ntiles, "tile list")) ||
+        (rc = dalloc(&d_list[1], ntiles, "tile list")) ||
+        (rc = dalloc(&d_flag, ntiles, "tile flags")) ||
+        (rc = dalloc(&d_dirty, ntiles, "tile flags")) ||
+        (rc = dalloc(&d_seed, ntiles, "tile seeds")) ||
+        (rc = dalloc(&d_jf0, ntiles, "jump flooding")) ||
+        (rc = dalloc(&d_jf1, ntiles, "jump flooding")) ||
+        (rc = dalloc(&d_tdist, ntiles, "tile distance")) ||
+        (rc = dalloc(&d_hist, nbins, "tile histogram")) ||
+        (rc = dalloc(&d_order, ntiles, "tile order")))
+      return rc;
+    JFC_HIP(hipMemcpy(d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpy(d_tile_inst, tile_inst.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpy(d_color_tiles, flat.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+    return JFC_OK;
+  }
+};
+
+// ---- accessors for the data-term file ---------------------------------------------
+
+namespace jfc {
+int model_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int **c0,
+                   long long *stride) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  if (inst < 0 || inst >= (int)g->inst.size())
+    return fail(JFC_ERR_INVALID, "instance index out of range");
+  *I = g->inst[inst];
+  *cs = g->cs;
+  *ct = g->ct;
+  *c0 = g->c0;
+  *stride = g->Ntot;
+  return JFC_OK;
+}
+
+void graph_mark_inputs(jfc_graph *g, int inst) {
+  g->caps_set[inst] = ~0;
+  g->solved = false;
+}
+}  // namespace jfc
+
+// ---- lifetime -------------------------------------------------------------------
+
+extern "C" const char *jfc_last_error(void) { return g_err.c_str(); }
+
+extern "C" void jfc_default_options(jfc_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_sweeps = 1 << 20;
+  o->check_every = 4;
+  o->relabel_work = 4.0;
+  o->max_local_iters = 64;
+  o->tile_shape = JFC_TILE_DEFAULT;
+  o->profile = 0;
+  o->stream = nullptr;
+}
+
+static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
+  if (!out) return fail(JFC_ERR_INVALID, "null graph pointer");
+  *out = nullptr;
+  if (count <= 0 || !shapes) return fail(JFC_ERR_INVALID, "empty batch");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(JFC_ERR_DEVICE, "no HIP device available");
+  }
+  jfc_graph *g = new jfc_graph();
+  long long off = 0;
+  g->dim = shapes[2] > 1 ? 3 : 2;
+  g->Kg = 0;
+  for (int i = 0; i < count; ++i) {
+    const int nx = shapes[4 * i], ny = shapes[4 * i + 1], nz = shapes[4 * i + 2],
+              conn = shapes[4 * i + 3];
+    const int dim = nz > 1 ? 3 : 2;
+    bool ok = nx > 0 && ny > 0 && nz > 0 && dim == g->dim &&
+              ((dim == 2 && (conn == 4 || conn == 8)) || (dim == 3 && conn == 6));
+    if (dim == 3 && conn != 6) ok = false;
+    if (!ok) {
+      delete g;
+      return fail(JFC_ERR_INVALID, "instance " + std::to_string(i) +
+                                       ": unsupported shape/connectivity (2D: 4 or 8, 3D: 6; "
+                                       "all instances of one graph share the dimensionality)");
+    }
+    InstDesc I{};
+    I.nx = nx; I.ny = ny; I.nz = nz; I.conn = conn; I.K = conn;
+    I.node_off = off;
+    off += (long long)nx * ny * nz;
+    g->inst.push_back(I);
+    g->Kg = std::max(g->Kg, conn);
+  }
+  if (off >= (1LL << 40)) { delete g; return fail(JFC_ERR_INVALID, "graph too large"); }
+  g->Ntot = off;
+  g->caps_set.assign(count, 0);
+  g->flows.assign(count, 0);
+  int rc;
+  const size_t N = (size_t)off;
+  if ((rc = dalloc(&g->cs, N, "source capacities")) ||
+      (rc = dalloc(&g->ct, N, "sink capacities")) ||
+      (rc = dalloc(&g->c0, N * g->Kg, "neighbour capacities")) ||
+      (rc = dalloc(&g->cap, N * g->Kg, "residual capacities")) ||
+      (rc = dalloc(&g->e, N, "excess")) || (rc = dalloc(&g->h, N, "heights")) ||
+      (rc = dalloc(&g->lab, N, "labels")) || (rc = dalloc(&g->d_ctr, 8, "queues")) ||
+      (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
+      (rc = dalloc(&g->d_total, 4, "counters")) ||
+      (rc = dalloc(&g->d_flagword, 4, "flags"))) {
+    delete g;
+    return rc;
+  }
+  launch_fill_i32(g->cs, off, 0, 0);
+  launch_fill_i32(g->ct, off, 0, 0);
+  launch_fill_i32(g->c0, off * g->Kg, 0, 0);
+  launch_fill_u8(g->lab, off, 0, 0);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    g->nwg = std::max(64, prop.multiProcessorCount * 8);
+  if ((rc = g->set_shape(default_shape(g->dim)))) { delete g; return rc; }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete g;
+    return fail(JFC_ERR_DEVICE, "device initialisation failed");
+  }
+  *out = g;
+  return JFC_OK;
+}
+
+extern "C" int jfc_create(jfc_graph **g, int dims, int nx, int ny, int nz,
+                          int connectivity) {
+  if (dims == 2 && nz != 1) return fail(JFC_ERR_INVALID, "2D graph needs nz == 1");
+  if (dims == 3 && nz < 2) return fail(JFC_ERR_INVALID, "3D graph needs nz >= 2");
+  if (dims != 2 && dims != 3) return fail(JFC_ERR_INVALID, "dims must be 2 or 3");
+  const int32_t s[4] = {nx, ny, nz, connectivity};
+  return create_impl(g, 1, s);
+}
+
+extern "C" int jfc_create_batch(jfc_graph **g, int count, const int32_t *shapes) {
+  return create_impl(g, count, shapes);
+}
+
+extern "C" void jfc_destroy(jfc_graph *g) {
+  if (!g) return;
+  (void)hipDeviceSynchronize();
+  delete g;
+}
+
+extern "C" int jfc_num_instances(const jfc_graph *g) {
+  return g ? (int)g->inst.size() : 0;
+}
+
+// ---- inputs ------------------------------------------------------------------------
+
+static int check_inst(jfc_graph *g, int inst) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  if (inst < 0 || inst >= (int)g->inst.size())
+    return fail(JFC_ERR_INVALID, "instance index out of range");
+  return JFC_OK;
+}
+
+static int put_plane(int *dst, const int32_t *src, long long n, int on_device) {
+  JFC_HIP(hipMemcpy(dst, src, n * sizeof(int32_t),
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  return JFC_OK;
+}
+
+static int validate(jfc_graph *g, const int *p, long long n, int lo, int hi,
+                    const char *what) {
+  launch_fill_i32(g->d_flagword + 1, 1, 0, 0);
+  launch_validate(p, n, lo, hi, g->d_flagword + 1, 0);
+  int bad = 0;
+  JFC_HIP(hipMemcpy(&bad, g->d_flagword + 1, sizeof(int), hipMemcpyDeviceToHost));
+  if (bad)
+    return fail(JFC_ERR_OVERFLOW, std::string(what) + " outside [" + std::to_string(lo) +
+                                      ", " + std::to_string(hi) + "]");
+  return JFC_OK;
+}
+
+extern "C" int jfc_set_terminal_caps(jfc_graph *g, int inst, const int32_t *cap_source,
+                                     const int32_t *cap_sink, int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!cap_source || !cap_sink) return fail(JFC_ERR_INVALID, "null capacity pointer");
+  const long long n = g->inst_nodes(inst);
+  const long long o = g->inst[inst].node_off;
+  if ((rc = put_plane(g->cs + o, cap_source, n, on_device)) ||
+      (rc = put_plane(g->ct + o, cap_sink, n, on_device)))
+    return rc;
+  if ((rc = validate(g, g->cs + o, n, 0, JFC_MAX_TERMINAL_CAP, "source capacity")) ||
+      (rc = validate(g, g->ct + o, n, 0, JFC_MAX_TERMINAL_CAP, "sink capacity"))) {
+    launch_fill_i32(g->cs + o, n, 0, 0);
+    launch_fill_i32(g->ct + o, n, 0, 0);
+    (void)hipDeviceSynchronize();
+    return rc;
+  }
+  g->caps_set[inst] |= 1;
+  g->solved = false;
+  return JFC_OK;
+}
+
+extern "C" int jfc_set_excess(jfc_graph *g, int inst, const int32_t *ef, int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!ef) return fail(JFC_ERR_INVALID, "null excess pointer");
+  const long long n = g->inst_nodes(inst);
+  const long long o = g->inst[inst].node_off;
+  int *tmp = nullptr;
+  if ((rc = dalloc(&tmp, n, "excess staging"))) return rc;
+  rc = put_plane(tmp, ef, n, on_device);
+  if (!rc) rc = validate(g, tmp, n, -JFC_MAX_TERMINAL_CAP, JFC_MAX_TERMINAL_CAP, "excess");
+  if (!rc) {
+    launch_excess_split(tmp, g->cs + o, g->ct + o, n, 0);
+    if (hipDeviceSynchronize() != hipSuccess) rc = fail(JFC_ERR_DEVICE, "excess split failed");
+  }
+  dfree(tmp);
+  if (rc) return rc;
+  g->caps_set[inst] |= 1;
+  g->solved = false;
+  return JFC_OK;
+}
+
+extern "C" int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int32_t *cap,
+                                     int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!cap) return fail(JFC_ERR_INVALID, "null capacity pointer");
+  if (arc < 0 || arc >= g->inst[inst].K)
+    return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
+  const long long n = g->inst_nodes(inst);
+  int *dst = g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off;
+  if ((rc = put_plane(dst, cap, n, on_device))) return rc;
+  if ((rc = validate(g, dst, n, 0, JFC_MAX_NEIGHBOR_CAP, "neighbour capacity"))) {
+    launch_fill_i32(dst, n, 0, 0);
+    (void)hipDeviceSynchronize();
+    return rc;
+  }
+  launch_zero_border(g->shape, g->inst[inst], g->c0 + (long long)arc * g->Ntot, arc, 0);
+  JFC_HIP(hipDeviceSynchronize());
+  g->caps_set[inst] |= 2 << arc;
+  g->solved = false;
+  return JFC_OK;
+}
+
+extern "C" int jfc_set_seeds(jfc_graph *g, int inst, const uint8_t *fg, const uint8_t *bg,
+                             int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  const long long n = g->inst_nodes(inst);
+  const long long o = g->inst[inst].node_off;
+  const unsigned char *dfg = fg, *dbg = bg;
+  unsigned char *tfg = nullptr, *tbg = nullptr;
+  if (!on_device) {
+    if (fg) {
+      if ((rc = dalloc(&tfg, n, "seed staging"))) return rc;
+      JFC_HIP(hipMemcpy(tfg, fg, n, hipMemcpyHostToDevice));
+      dfg = tfg;
+    }
+    if (bg) {
+      if ((rc = dalloc(&tbg, n, "seed staging"))) { dfree(tfg); return rc; }
+      JFC_HIP(hipMemcpy(tbg, bg, n, hipMemcpyHostToDevice));
+      dbg = tbg;
+    }
+  }
+  launch_seeds(g->cs + o, g->ct + o, dfg, dbg, n, 0);
+  hipError_t err = hipDeviceSynchronize();
+  dfree(tfg);
+  dfree(tbg);
+  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
+  g->caps_set[inst] |= 1;
+  g->solved = false;
+  return JFC_OK;
+}
+
+// ---- solve -------------------------------------------------------------------------------
+
+namespace {
+
+struct Timer {
+  std::vector<hipEvent_t> *pool;
+  size_t used = 0;
+  hipStream_t st;
+  hipEvent_t next() {
+    if (used == pool->size()) {
+      hipEvent_t ev;
+      if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+      pool->push_back(ev);
+    }
+    return (*pool)[used++];
+  }
+  hipEvent_t mark() {
+    hipEvent_t ev = next();
+    if (ev) (void)hipEventRecord(ev, st);
+    return ev;
+  }
+  static double ms(hipEvent_t a, hipEvent_t b) {
+    float f = 0.f;
+    if (a && b && hipEventElapsedTime(&f, a, b) == hipSuccess) return f;
+    return 0.0;
+  }
+};
+
+struct Solver {
+  jfc_graph *g;
+  jfc_options o;
+  jfc_stats s;
+  hipStream_t st;
+  Timer tm;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> push_ev, gr_ev, cut_ev;
+
+  Queue queue_from_list() {
+    Queue q{g->d_list[g->slot], g->d_ctr, g->slot, -1};
+    return q;
+  }
+  void advance() { g->slot ^= 1; }
+
+  int read_int(const int *dptr, int *out) {
+    JFC_HIP(hipMemcpyAsync(out, dptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipStreamSynchronize(st));
+    return JFC_OK;
+  }
+
+  // Tile passes of a monotone search (relabel or cut) until no tile is dirty.
+  // The first pass visits every tile in jump-flooding distance order; later
+  // passes visit the dirty tiles, alternating direction (back-and-forth,
+  // QGCWidget.cpp:2221-2240).
+  int search(bool cut, int *passes, long long *tiles) {
+    const Tiles T = g->tiles();
+    const Planes P = g->planes();
+    launch_tile_order(g->shape, T, g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist, g->d_hist,
+                      g->nbins, g->d_order, g->max_tile_dim, nullptr, st);
+    launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
+    Queue q{g->d_order, g->d_ctr, g->slot, g->ntiles};
+    const int nwg = std::min(g->nwg, g->ntiles);
+    if (cut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
+    else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, st);
+    advance();
+    *passes += 1;
+    *tiles += g->ntiles;
+    for (int pass = 1;; ++pass) {
+      Queue qc = queue_from_list();
+      launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, qc, nullptr,
+                     g->d_total + (cut ? 2 : 1), st);
+      int cnt = 0;
+      int rc = read_int(g->d_ctr + 2 * g->slot, &cnt);
+      if (rc) return rc;
+      if (cnt == 0) break;
+      const int w = std::min(g->nwg, cnt);
+      if (cut) launch_reach(g->shape, w, T, P, qc, g->d_dirty, g->d_flagword, st);
+      else launch_relax(g->shape, w, T, P, qc, g->d_dirty, st);
+      advance();
+      *passes += 1;
+      *tiles += cnt;
+      if (pass > 1000000) return fail(JFC_ERR_STATE, "search did not settle");
+    }
+    return JFC_OK;
+  }
+
+  int global_relabel() {
+    hipEvent_t a = tm.mark();
+    launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
+    long long tiles = 0;
+    int rc = search(false, &s.relabel_passes, &tiles);
+    if (rc) return rc;
+    s.relabel_tiles += tiles;
+    launch_count_active(g->shape, g->tiles(), g->planes(), g->d_flag, st);
+    gr_ev.push_back({a, tm.mark()});
+    s.global_relabels++;
+    return JFC_OK;
+  }
+
+  // Returns 1 in *found when an augmenting path still exists.
+  int canonical_cut(int *found) {
+    hipEvent_t a = tm.mark();
+    launch_fill_i32(g->d_flagword, 1, 0, st);
+    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
+    long long tiles = 0;
+    int rc = search(true, &s.cut_passes, &tiles);
+    if (rc) return rc;
+    s.cut_tiles += tiles;
+    cut_ev.push_back({a, tm.mark()});
+    return read_int(g->d_flagword, found);
+  }
+
+  int run() {
+    const Tiles T = g->tiles();
+    const Planes P = g->planes();
+    const int nI = (int)g->inst.size();
+    // reset the residual state from the inputs
+    if (g->sweep_cap < o.max_sweeps + 1) {
+      dfree(g->d_sweep);
+      int rc = dalloc(&g->d_sweep, (size_t)o.max_sweeps + 1, "sweep counters");
+      if (rc) return rc;
+      g->sweep_cap = o.max_sweeps + 1;
+    }
+    hipEvent_t t0 = tm.mark();
+    launch_fill_i32(g->d_ctr, 8, 0, st);
+    launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
+    launch_fill_i32((int *)g->d_total, 8, 0, st);
+    launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
+    g->slot = 0;
+    JFC_HIP(hipMemcpyAsync(g->cap, g->c0, (size_t)g->Ntot * g->Kg * sizeof(int),
+                           hipMemcpyDeviceToDevice, st));
+    launch_fold(g->shape, T, P, g->d_red, st);
+    std::vector<long long> red((size_t)nI * RED_SLOTS);
+    JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
+                           hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nI; ++i)
+      if (red[i * RED_SLOTS + RED_DEF0] == 0 && red[i * RED_SLOTS + RED_SUP0] == 0 &&
+          red[i * RED_SLOTS + RED_CONST] == 0)
+        return fail(JFC_ERR_NO_SEEDS, "instance " + std::to_string(i) +
+                                          " has no terminal capacity (no labels)");
+    int rc = global_relabel();
+    if (rc) return rc;
+    long long total_at_gr = 0;
+    int since_check = 0;
+    int cert_rounds = 0;
+    bool done = false;
+    int sweep = 0;
+    while (!done) {
+      bool active = true;
+      while (sweep < o.max_sweeps) {
+        for (int c = 0; c < g->ncolors; ++c) {
+          Queue q = queue_from_list();
+          const int n = g->color_off[c + 1] - g->color_off[c];
+          launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, q,
+                         g->d_sweep + sweep, g->d_total, st);
+          hipEvent_t a = o.profile ? tm.mark() : nullptr;
+          launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
+                      o.max_local_iters, st);
+          if (o.profile) push_ev.push_back({a, tm.mark()});
+          advance();
+          s.push_launches++;
+        }
+        ++sweep;
+        if (++since_check >= o.check_every || sweep == o.max_sweeps) {
+          since_check = 0;
+          int cnt = 0;
+          long long total = 0;
+          JFC_HIP(hipMemcpyAsync(&cnt, g->d_sweep + sweep - 1, sizeof(int),
+                                 hipMemcpyDeviceToHost, st));
+          JFC_HIP(hipMemcpyAsync(&total, g->d_total, sizeof(long long),
+                                 hipMemcpyDeviceToHost, st));
+          JFC_HIP(hipStreamSynchronize(st));
+          if (cnt == 0) { active = false; break; }
+          if (o.relabel_work > 0 &&
+              (double)(total - total_at_gr) > o.relabel_work * g->ntiles) {
+            if ((rc = global_relabel())) return rc;
+            total_at_gr = total;
+          }
+        }
+      }
+      s.sweeps = sweep;
+      if (active) break;  // sweep budget exhausted
+      int found = 0;
+      if ((rc = canonical_cut(&found))) return rc;
+      if (!found) { done = true; break; }
+      // an augmenting path survived: relabel exactly and continue
+      s.certificate_failures++;
+      if (++cert_rounds > 16) return fail(JFC_ERR_STATE, "certificate kept failing");
+      if ((rc = global_relabel())) return rc;
+    }
+    if (!done) {
+      // labels of the current preflow are still useful to callers
+      int found = 0;
+      if ((rc = canonical_cut(&found))) return rc;
+    }
+    launch_deficit(g->shape, T, P, g->d_red, st);
+    hipEvent_t t1 = tm.mark();
+    JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
+                           hipMemcpyDeviceToHost, st));
+    long long totals[4] = {0, 0, 0, 0};
+    JFC_HIP(hipMemcpyAsync(totals, g->d_total, sizeof(totals), hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nI; ++i)
+      g->flows[i] = red[i * RED_SLOTS + RED_CONST] + red[i * RED_SLOTS + RED_DEF0] -
+                    red[i * RED_SLOTS + RED_DEF];
+    s.flow = g->flows[0];
+    s.converged = done ? 1 : 0;
+    s.push_tiles = totals[0];
+    s.relabel_tiles += 0;
+    s.push_bytes = totals[0] * 256LL * push_bytes_per_node(g->Kg);
+    s.ms_total = Timer::ms(t0, t1);
+    for (auto &p : push_ev) s.ms_push += Timer::ms(p.first, p.second);
+    for (auto &p : gr_ev) s.ms_relabel += Timer::ms(p.first, p.second);
+    for (auto &p : cut_ev) s.ms_cut += Timer::ms(p.first, p.second);
+    g->solved = true;
+    return done ? JFC_OK : JFC_NOT_CONVERGED;
+  }
+};
+
+}  // namespace
+
+extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  Solver S{};
+  S.g = g;
+  if (opt) S.o = *opt;
+  else jfc_default_options(&S.o);
+  if (S.o.max_sweeps <= 0) S.o.max_sweeps = 1 << 20;
+  if (S.o.max_sweeps > (1 << 24)) S.o.max_sweeps = 1 << 24;
+  if (S.o.check_every <= 0) S.o.check_every = 1;
+  if (S.o.max_local_iters <= 0) S.o.max_local_iters = 1;
+  const int sh = shape_from_option(S.o.tile_shape, g->dim);
+  if (!sh) return fail(JFC_ERR_INVALID, "tile shape does not match the graph's dimensionality");
+  int rc = g->set_shape(sh);
+  if (rc) return rc;
+  std::memset(&S.s, 0, sizeof(S.s));
+  S.st = (hipStream_t)S.o.stream;
+  S.tm.pool = &g->ev_pool;
+  S.tm.st = S.st;
+  g->solved = false;
+  rc = S.run();
+  if (stats) *stats = S.s;
+  return rc;
+}
+
+// ---- outputs -----------------------------------------------------------------------------
+
+extern "C" int jfc_get_flow(jfc_graph *g, int inst, int64_t *flow) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!flow) return fail(JFC_ERR_INVALID, "null output");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  *flow = g->flows[inst];
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_flows(jfc_graph *g, int64_t *flows) {
+  if (!g || !flows) return fail(JFC_ERR_INVALID, "null argument");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  for (size_t i = 0; i < g->flows.size(); ++i) flows[i] = g->flows[i];
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!labels) return fail(JFC_ERR_INVALID, "null output");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  JFC_HIP(hipMemcpy(labels, g->lab + g->inst[inst].node_off, g->inst_nodes(inst),
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!energy) return fail(JFC_ERR_INVALID, "null output");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  const int nI = (int)g->inst.size();
+  std::vector<long long> red((size_t)nI * RED_SLOTS);
+  JFC_HIP(hipMemcpy(red.data(), g->d_red, red.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nI; ++i) red[i * RED_SLOTS + RED_ENERGY] = 0;
+  JFC_HIP(hipMemcpy(g->d_red, red.data(), red.size() * sizeof(long long), hipMemcpyHostToDevice));
+  launch_energy(g->shape, g->tiles(), g->planes(), g->d_red, 0);
+  JFC_HIP(hipMemcpy(red.data(), g->d_red, red.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  *energy = red[inst * RED_SLOTS + RED_ENERGY];
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_terminal_caps(jfc_graph *g, int inst, int32_t *cap_source,
+                                     int32_t *cap_sink, int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  const long long n = g->inst_nodes(inst), o = g->inst[inst].node_off;
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (cap_source) JFC_HIP(hipMemcpy(cap_source, g->cs + o, n * 4, k));
+  if (cap_sink) JFC_HIP(hipMemcpy(cap_sink, g->ct + o, n * 4, k));
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *cap,
+                                     int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!cap) return fail(JFC_ERR_INVALID, "null output");
+  if (arc < 0 || arc >= g->inst[inst].K)
+    return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
+  const long long n = g->inst_nodes(inst);
+  JFC_HIP(hipMemcpy(cap, g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off, n * 4,
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+
+extern "C" int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims,
+                               int32_t *grid_dims, int32_t *tile_off) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  int tw, th, td;
+  shape_dims(g->shape, &tw, &th, &td);
+  if (ntiles) *ntiles = g->ntiles;
+  if (tile_dims) { tile_dims[0] = tw; tile_dims[1] = th; tile_dims[2] = td; }
+  const InstDesc &I = g->inst[inst];
+  if (grid_dims) { grid_dims[0] = I.tx; grid_dims[1] = I.ty; grid_dims[2] = I.tz; }
+  if (tile_off) *tile_off = I.tile_off;
+  return JFC_OK;
+}
+
+extern "C" int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int32_t *tile_dist,
+                                    int32_t *order, int32_t *max_dist) {
+  if (!g || !tile_seed) return fail(JFC_ERR_INVALID, "null argument");
+  JFC_HIP(hipMemcpy(g->d_seed, tile_seed, g->ntiles, hipMemcpyHostToDevice));
+  launch_tile_order(g->shape, g->tiles(), g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist,
+                    g->d_hist, g->nbins, g->d_order, g->max_tile_dim, g->d_flagword + 2, 0);
+  JFC_HIP(hipDeviceSynchronize());
+  if (tile_dist) JFC_HIP(hipMemcpy(tile_dist, g->d_tdist, g->ntiles * 4, hipMemcpyDeviceToHost));
+  if (order) JFC_HIP(hipMemcpy(order, g->d_order, g->ntiles * 4, hipMemcpyDeviceToHost));
+  if (max_dist) JFC_HIP(hipMemcpy(max_dist, g->d_flagword + 2, 4, hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+
+extern "C" int jfc_backward(jfc_graph *g, int inst, float *d_cap_source, float *d_cap_sink,
+                            float *d_neighbor, int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  const InstDesc &I = g->inst[inst];
+  const long long n = g->inst_nodes(inst);
+  if (on_device) {
+    launch_backward(g->shape, g->tiles(), g->planes(), I, d_cap_source, d_cap_sink,
+                    d_neighbor, 0);
+    JFC_HIP(hipDeviceSynchronize());
+    return JFC_OK;
+  }
+  float *a = nullptr, *b = nullptr, *c = nullptr;
+  if ((d_cap_source && (rc = dalloc(&a, n, "gradient"))) ||
+      (d_cap_sink && (rc = dalloc(&b, n, "gradient"))) ||
+      (d_neighbor && (rc = dalloc(&c, n * I.K, "gradient")))) {
+    dfree(a); dfree(b); dfree(c);
+    return rc;
+  }
+  launch_backward(g->shape, g->tiles(), g->planes(), I, a, b, c, 0);
+  hipError_t err = hipDeviceSynchronize();
+  if (err == hipSuccess && a) err = hipMemcpy(d_cap_source, a, n * 4, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && b) err = hipMemcpy(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && c) err = hipMemcpy(d_neighbor, c, n * I.K * 4, hipMemcpyDeviceToHost);
+  dfree(a); dfree(b); dfree(c);
+  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
+  return JFC_OK;
+}

@@ -1,0 +1,115 @@
+// jfc_internal.h -- structures shared by the host driver and the gfx950
+// kernels of libjfcut.  Nothing here is part of the C-ABI (include/jfcut.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace jfc {
+
+// Height of a node proven unable to reach any deficit node.
+constexpr int HINF = 0x3fffffff;
+
+// Per-instance reduction slots (int64): folded constant, initial deficit,
+// final deficit, energy, initial supply.
+constexpr int RED_CONST = 0, RED_DEF0 = 1, RED_DEF = 2, RED_ENERGY = 3,
+              RED_SUP0 = 4, RED_SLOTS = 5;
+
+// One grid instance inside a graph.  All instances share the plane arrays;
+// instance i owns nodes [node_off, node_off + nx*ny*nz) and tiles
+// [tile_off, tile_off + tx*ty*tz).
+struct InstDesc {
+  int nx, ny, nz;
+  int conn, K;
+  int tx, ty, tz;
+  int tile_off;
+  int ntiles;
+  long long node_off;
+};
+
+struct Planes {
+  int *e;            // excess (> 0) / deficit (< 0), folded terminal arcs
+  int *h;            // height; HINF = cannot reach a deficit
+  int *cap;          // K residual planes, plane d at cap + d * stride
+  const int *c0;     // input neighbour planes (same layout)
+  const int *cs;     // input source capacities
+  const int *ct;     // input sink capacities
+  unsigned char *lab;// canonical labels (1 = source side)
+  long long stride;  // nodes per plane (all instances)
+};
+
+struct Tiles {
+  const InstDesc *inst;
+  const int *tile_inst;  // instance of every tile
+  int ntiles;
+};
+
+// Tile shapes (256 nodes each, one node per lane).
+enum ShapeId {
+  SH_32x8 = 1,
+  SH_64x4 = 2,
+  SH_16x16 = 3,
+  SH_8x8x4 = 4,
+  SH_32x4x2 = 5
+};
+bool shape_is_3d(int shape);
+void shape_dims(int shape, int *tw, int *th, int *td);
+
+// Persistent work queues: ctr[2*slot] = count, ctr[2*slot+1] = head.  Every
+// consumer zeroes the other slot, so producer/consumer launches alternate
+// slots without any memset.
+struct Queue {
+  int *list;
+  int *ctr;
+  int slot;
+  int fixed_count;  // >= 0: consume list[0..fixed_count) instead of ctr count
+};
+
+// ---- launchers (all asynchronous on `st`) --------------------------------
+void launch_fill_i32(int *p, long long n, int v, hipStream_t st);
+void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t st);
+
+void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
+                 hipStream_t st);
+void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
+                     hipStream_t st);
+void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
+                        hipStream_t st);
+void launch_seeds(int *cs, int *ct, const unsigned char *fg,
+                  const unsigned char *bg, long long n, hipStream_t st);
+void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
+                         hipStream_t st);
+
+void launch_gr_init(int shape, const Tiles &T, const Planes &P,
+                    unsigned char *seed, hipStream_t st);
+void launch_cut_init(int shape, const Tiles &T, const Planes &P,
+                     unsigned char *seed, hipStream_t st);
+void launch_count_active(int shape, const Tiles &T, const Planes &P,
+                         unsigned char *flag, hipStream_t st);
+
+// Jump-flooding tile distance + counting sort of tiles by distance.
+void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
+                       int *jf0, int *jf1, int *tdist, int *hist, int nbins,
+                       int *order, int max_tile_dim, int *max_dist,
+                       hipStream_t st);
+
+void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
+                    const Queue &q, int *sweep_cnt, long long *total,
+                    hipStream_t st);
+void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
+                 const Queue &q, unsigned char *flag, int max_iters,
+                 hipStream_t st);
+void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
+                  const Queue &q, unsigned char *dirty, hipStream_t st);
+void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
+                  const Queue &q, unsigned char *dirty, int *cert,
+                  hipStream_t st);
+void launch_deficit(int shape, const Tiles &T, const Planes &P, long long *red,
+                    hipStream_t st);
+void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
+                   hipStream_t st);
+void launch_backward(int shape, const Tiles &T, const Planes &P,
+                     const InstDesc &I, float *dcs, float *dct, float *dnb,
+                     hipStream_t st);
+
+}  // namespace jfc

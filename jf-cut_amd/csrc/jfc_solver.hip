@@ -1,0 +1,929 @@
+// jfc_solver.hip -- gfx950 kernels of the grid max-flow solve.
+//
+// Reference counterparts (paths relative to src/QVisualizer/):
+//   k_push     graphcut_relabel_push   cl/GraphCut/graphcut_push_relabel.cl:25-203 (2D), :215-439 (3D)
+//   k_relax    graphcut_global_relabel cl/GraphCut/graphcut_global_relabel.cl:50-124, :128-249
+//   k_gr_init  graphcut_init_global_relabel  graphcut_global_relabel.cl:19-48
+//   k_reach    graphcut_compute_cut    cl/GraphCut/graphcut_compute_cut.cl:51-140, :182-282
+//   k_cut_init graphcut_init_cut       graphcut_compute_cut.cl:19-49
+//   k_count_active / k_compact  graphcut_count_node + clogs scan + graphcut_scatter_node
+//                               cl/GraphCut/graphcut_count_block.cl:19-65
+//   k_jf_*     graphcut_bfs_jump_flooding / _histogram / _mapping
+//              cl/GraphCut/graphcut_bfs_block.cl:43-91 (+ reduce_histogram, INF reduce.cl:56-141)
+//   k_deficit  reduce_flow  bin/x64/cl/Infrastructures/reduce.cl:17-42 (int64 here)
+//
+// Layout: every graph stores int32 planes (e, h, K residual planes) in
+// structure-of-arrays order over all instances.  Kernels work on tiles of
+// 256 nodes (one node per lane) staged in LDS with a one-node halo; tiles of
+// one colour class never share a node or halo cell, so a push-relabel launch
+// over one colour class needs no global atomics except for halo excess.
+#include "jfc_device.h"
+
+#include <climits>
+
+namespace jfc {
+
+bool shape_is_3d(int s) { return s == SH_8x8x4 || s == SH_32x4x2; }
+
+void shape_dims(int s, int *tw, int *th, int *td) {
+  int a = 0, b = 0, c = 0;
+  JFC_DISPATCH(s, { a = S::TW; b = S::TH; c = S::TD; });
+  *tw = a; *th = b; *td = c;
+}
+
+// ---- small utilities --------------------------------------------------------
+
+__global__ void k_fill_i32(int *p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u8(unsigned char *p, long long n, unsigned char v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static int grid_for(long long n, int per_block = 256, int cap = 8192) {
+  long long g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+void launch_fill_i32(int *p, long long n, int v, hipStream_t st) {
+  if (n > 0) k_fill_i32<<<grid_for(n), 256, 0, st>>>(p, n, v);
+}
+void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t st) {
+  if (n > 0) k_fill_u8<<<grid_for(n), 256, 0, st>>>(p, n, v);
+}
+
+__global__ void k_validate(const int *p, long long n, int lo, int hi, int *bad) {
+  int b = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int v = p[i];
+    b |= (v < lo) | (v > hi);
+  }
+  if (__ballot(b) && lane_id() == 0) atomicOr(bad, 1);
+}
+
+void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
+                     hipStream_t st) {
+  if (n > 0) k_validate<<<grid_for(n), 256, 0, st>>>(p, n, lo, hi, bad);
+}
+
+__global__ void k_seeds(int *cs, int *ct, const unsigned char *fg,
+                        const unsigned char *bg, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (fg && fg[i]) { cs[i] = 1 << 30; ct[i] = 0; }
+    else if (bg && bg[i]) { cs[i] = 0; ct[i] = 1 << 30; }
+  }
+}
+
+void launch_seeds(int *cs, int *ct, const unsigned char *fg,
+                  const unsigned char *bg, long long n, hipStream_t st) {
+  if (n > 0) k_seeds<<<grid_for(n), 256, 0, st>>>(cs, ct, fg, bg, n);
+}
+
+__global__ void k_excess_split(const int *ef, int *cs, int *ct, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int v = ef[i];
+    cs[i] = v > 0 ? v : 0;
+    ct[i] = v < 0 ? -v : 0;
+  }
+}
+
+void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
+                         hipStream_t st) {
+  if (n > 0) k_excess_split<<<grid_for(n), 256, 0, st>>>(ef, cs, ct, n);
+}
+
+// Zero the arcs of plane d that leave the grid of one instance.
+template <class S>
+__global__ void k_zero_border(InstDesc I, int *plane, int d) {
+  int dx, dy, dz;
+  dir_of<S::D3>(d, dx, dy, dz);
+  const long long n = (long long)I.nx * I.ny * I.nz;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i % I.nx);
+    const long long r = i / I.nx;
+    const int y = (int)(r % I.ny);
+    const int z = (int)(r / I.ny);
+    const int X = x + dx, Y = y + dy, Z = z + dz;
+    if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz)
+      plane[I.node_off + i] = 0;
+  }
+}
+
+void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
+                        hipStream_t st) {
+  const long long n = (long long)I.nx * I.ny * I.nz;
+  JFC_DISPATCH(shape, (k_zero_border<S><<<grid_for(n), 256, 0, st>>>(I, plane, d)));
+}
+
+// ---- block reductions --------------------------------------------------------
+
+__device__ __forceinline__ long long block_sum_ll(long long v, long long *buf) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane_id() == 0) buf[w] = v;
+  __syncthreads();
+  long long r = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += buf[i];
+  return r;
+}
+
+// Tile-parallel kernels: workgroup b walks a contiguous chunk of tiles so that
+// per-instance sums are flushed once per instance change.
+struct Chunk {
+  int t0, t1;
+  __device__ __forceinline__ Chunk(int ntiles) {
+    const int per = (ntiles + gridDim.x - 1) / gridDim.x;
+    t0 = blockIdx.x * per;
+    t1 = min(ntiles, t0 + per);
+  }
+};
+
+static int tile_grid(int ntiles) { return ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096; }
+
+__device__ __forceinline__ void fold_flush(long long *red, int inst, long long a,
+                                           long long b, long long c, long long *buf) {
+  const long long A = block_sum_ll(a, buf);
+  const long long B = block_sum_ll(b, buf);
+  const long long C = block_sum_ll(c, buf);
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_CONST], (unsigned long long)A);
+    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_DEF0], (unsigned long long)B);
+    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_SUP0], (unsigned long long)C);
+  }
+}
+
+// Fold terminal arcs: e = cs - ct; const += min(cs, ct);
+// deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
+template <class S>
+__global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red) {
+  __shared__ long long buf[4];
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  int cur = -1;
+  long long a = 0, b = 0, c = 0;
+  for (int t = C.t0; t < C.t1; ++t) {
+    const int inst = T.tile_inst[t];
+    if (inst != cur) {
+      if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
+      cur = inst;
+      a = b = c = 0;
+    }
+    const TileRef r = tile_ref<S>(T, t);
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
+      const long long v = node_index(r.I, gx, gy, gz);
+      const int s = P.cs[v], k = P.ct[v];
+      P.e[v] = s - k;
+      a += s < k ? s : k;
+      b += k > s ? k - s : 0;
+      c += s > k ? s - k : 0;
+    }
+  }
+  if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
+}
+
+void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
+                 hipStream_t st) {
+  JFC_DISPATCH(shape, (k_fold<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+}
+
+// Remaining deficit per instance: sum of max(-e, 0).
+template <class S>
+__global__ __launch_bounds__(256) void k_deficit(Tiles T, Planes P, long long *red) {
+  __shared__ long long buf[4];
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  int cur = -1;
+  long long a = 0;
+  for (int t = C.t0; t < C.t1; ++t) {
+    const int inst = T.tile_inst[t];
+    if (inst != cur) {
+      if (cur >= 0) {
+        const long long A = block_sum_ll(a, buf);
+        if (threadIdx.x == 0)
+          atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
+      }
+      cur = inst;
+      a = 0;
+    }
+    const TileRef r = tile_ref<S>(T, t);
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
+      const int e = P.e[node_index(r.I, gx, gy, gz)];
+      a += e < 0 ? -e : 0;
+    }
+  }
+  if (cur >= 0) {
+    const long long A = block_sum_ll(a, buf);
+    if (threadIdx.x == 0)
+      atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
+  }
+}
+
+void launch_deficit(int shape, const Tiles &T, const Planes &P, long long *red,
+                    hipStream_t st) {
+  JFC_DISPATCH(shape, (k_deficit<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+}
+
+// Cut energy on the input capacities (QGCWidget.cpp:2452-2520).
+template <class S>
+__global__ __launch_bounds__(256) void k_energy(Tiles T, Planes P, long long *red) {
+  __shared__ long long buf[4];
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  int cur = -1;
+  long long a = 0;
+  for (int t = C.t0; t < C.t1; ++t) {
+    const int inst = T.tile_inst[t];
+    if (inst != cur) {
+      if (cur >= 0) {
+        const long long A = block_sum_ll(a, buf);
+        if (threadIdx.x == 0)
+          atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_ENERGY], (unsigned long long)A);
+      }
+      cur = inst;
+      a = 0;
+    }
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    if (gx < I.nx && gy < I.ny && gz < I.nz) {
+      const long long v = node_index(I, gx, gy, gz);
+      if (P.lab[v]) {
+        a += P.ct[v];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) {
+          if (d >= I.K) break;
+          int dx, dy, dz;
+          dir_of<S::D3>(d, dx, dy, dz);
+          const int X = gx + dx, Y = gy + dy, Z = gz + dz;
+          if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) continue;
+          if (!P.lab[node_index(I, X, Y, Z)]) a += P.c0[d * P.stride + v];
+        }
+      } else {
+        a += P.cs[v];
+      }
+    }
+  }
+  if (cur >= 0) {
+    const long long A = block_sum_ll(a, buf);
+    if (threadIdx.x == 0)
+      atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_ENERGY], (unsigned long long)A);
+  }
+}
+
+void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
+                   hipStream_t st) {
+  JFC_DISPATCH(shape, (k_energy<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+}
+
+// Subgradient of the cut energy for instance `inst` (outputs in the
+// instance's own node numbering).
+template <class S>
+__global__ __launch_bounds__(256) void k_backward(Tiles T, Planes P, int t_begin,
+                                                  int t_count, float *dcs,
+                                                  float *dct, float *dnb) {
+  const Lane<S> L(threadIdx.x);
+  for (int t = t_begin + blockIdx.x; t < t_begin + t_count; t += gridDim.x) {
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    if (!(gx < I.nx && gy < I.ny && gz < I.nz)) continue;
+    const long long v = node_index(I, gx, gy, gz);
+    const long long lv = v - I.node_off;
+    const long long n = (long long)I.nx * I.ny * I.nz;
+    const bool s = P.lab[v] != 0;
+    if (dcs) dcs[lv] = s ? 0.f : 1.f;
+    if (dct) dct[lv] = s ? 1.f : 0.f;
+    if (dnb) {
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) {
+        if (d >= I.K) break;
+        int dx, dy, dz;
+        dir_of<S::D3>(d, dx, dy, dz);
+        const int X = gx + dx, Y = gy + dy, Z = gz + dz;
+        float g = 0.f;
+        if (s && X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
+          g = P.lab[node_index(I, X, Y, Z)] ? 0.f : 1.f;
+        dnb[d * n + lv] = g;
+      }
+    }
+  }
+}
+
+void launch_backward(int shape, const Tiles &T, const Planes &P,
+                     const InstDesc &I, float *dcs, float *dct, float *dnb,
+                     hipStream_t st) {
+  JFC_DISPATCH(shape, (k_backward<S><<<tile_grid(I.ntiles), 256, 0, st>>>(
+                          T, P, I.tile_off, I.ntiles, dcs, dct, dnb)));
+}
+
+// ---- initialisation of relabel / cut searches --------------------------------
+
+// graphcut_init_global_relabel: h = 0 at deficit nodes, unvisited elsewhere;
+// tiles holding a deficit node seed the jump flooding.
+template <class S>
+__global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  for (int t = C.t0; t < C.t1; ++t) {
+    const TileRef r = tile_ref<S>(T, t);
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    int s = 0;
+    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
+      const long long v = node_index(r.I, gx, gy, gz);
+      s = P.e[v] < 0;
+      P.h[v] = s ? 0 : HINF;
+    }
+    s = __syncthreads_or(s);
+    if (threadIdx.x == 0) seed[t] = (unsigned char)s;
+  }
+}
+
+void launch_gr_init(int shape, const Tiles &T, const Planes &P,
+                    unsigned char *seed, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_gr_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, seed)));
+}
+
+// graphcut_init_cut: label = 1 where excess > 0; tiles holding one seed the
+// jump flooding.
+template <class S>
+__global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned char *seed) {
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  for (int t = C.t0; t < C.t1; ++t) {
+    const TileRef r = tile_ref<S>(T, t);
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    int s = 0;
+    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
+      const long long v = node_index(r.I, gx, gy, gz);
+      s = P.e[v] > 0;
+      P.lab[v] = (unsigned char)s;
+    }
+    s = __syncthreads_or(s);
+    if (threadIdx.x == 0) seed[t] = (unsigned char)s;
+  }
+}
+
+void launch_cut_init(int shape, const Tiles &T, const Planes &P,
+                     unsigned char *seed, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_cut_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, seed)));
+}
+
+// graphcut_count_node: a tile is active if one of its nodes has excess and a
+// finite height.
+template <class S>
+__global__ __launch_bounds__(256) void k_count_active(Tiles T, Planes P,
+                                                      unsigned char *flag) {
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  for (int t = C.t0; t < C.t1; ++t) {
+    const TileRef r = tile_ref<S>(T, t);
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    int a = 0;
+    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
+      const long long v = node_index(r.I, gx, gy, gz);
+      a = P.e[v] > 0 && P.h[v] < HINF;
+    }
+    a = __syncthreads_or(a);
+    if (threadIdx.x == 0) flag[t] = (unsigned char)a;
+  }
+}
+
+void launch_count_active(int shape, const Tiles &T, const Planes &P,
+                         unsigned char *flag, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_count_active<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, flag)));
+}
+
+// ---- jump flooding over the tile grid + counting sort --------------------------
+
+__device__ __forceinline__ void tile_xyz(const InstDesc &I, int lt, int &x, int &y,
+                                         int &z) {
+  x = lt % I.tx;
+  const int q = lt / I.tx;
+  y = q % I.ty;
+  z = q / I.ty;
+}
+
+__global__ void k_jf_init(Tiles T, const unsigned char *seed, int *jf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.ntiles) return;
+  const int i = T.tile_inst[t];
+  jf[t] = seed[t] ? t - T.inst[i].tile_off : -1;
+}
+
+// One jump-flooding pass at jump `step`: keep the candidate seed with the
+// smallest taxicab distance (ties: smallest tile index) among the 3^D
+// neighbours at that jump (graphcut_bfs_block.cl:43-63).
+__global__ void k_jf_pass(Tiles T, const int *in, int *out, int step, int d3) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.ntiles) return;
+  const InstDesc I = T.inst[T.tile_inst[t]];
+  int x, y, z;
+  tile_xyz(I, t - I.tile_off, x, y, z);
+  int best = in[t], bd = INT_MAX;
+  if (best >= 0) {
+    int sx, sy, sz;
+    tile_xyz(I, best, sx, sy, sz);
+    bd = abs(sx - x) + abs(sy - y) + abs(sz - z);
+  }
+  const int zr = d3 ? 1 : 0;
+  for (int dz = -zr; dz <= zr; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        if (!dx && !dy && !dz) continue;
+        const int X = x + dx * step, Y = y + dy * step, Z = z + dz * step;
+        if (X < 0 || X >= I.tx || Y < 0 || Y >= I.ty || Z < 0 || Z >= I.tz) continue;
+        const int c = in[I.tile_off + X + I.tx * (Y + I.ty * Z)];
+        if (c < 0) continue;
+        int sx, sy, sz;
+        tile_xyz(I, c, sx, sy, sz);
+        const int d = abs(sx - x) + abs(sy - y) + abs(sz - z);
+        if (d < bd || (d == bd && c < best)) { bd = d; best = c; }
+      }
+  out[t] = best;
+}
+
+__global__ void k_jf_dist(Tiles T, const int *jf, int *tdist, int nbins) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.ntiles) return;
+  const InstDesc I = T.inst[T.tile_inst[t]];
+  const int c = jf[t];
+  int d = nbins - 1;
+  if (c >= 0) {
+    int x, y, z, sx, sy, sz;
+    tile_xyz(I, t - I.tile_off, x, y, z);
+    tile_xyz(I, c, sx, sy, sz);
+    d = min(abs(sx - x) + abs(sy - y) + abs(sz - z), nbins - 2);
+  }
+  tdist[t] = d;
+}
+
+// Largest finite tile distance (reduce_histogram, reduce.cl:56-141).
+__global__ void k_max_dist(const int *tdist, int n, int unreachable, int *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int d = (t < n && tdist[t] != unreachable) ? tdist[t] : 0;
+  for (int o = 32; o > 0; o >>= 1) d = max(d, __shfl_xor(d, o));
+  if (lane_id() == 0 && d > 0) atomicMax(out, d);
+}
+
+__global__ void k_hist(const int *tdist, int n, int *hist) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) atomicAdd(&hist[tdist[t]], 1);
+}
+
+// In-place exclusive scan of n ints by one 1024-lane workgroup (the clogs
+// Scan of QGCWidget.cpp:810, on a bin count small enough for one group).
+__global__ __launch_bounds__(1024) void k_scan_1wg(int *a, int n) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < n ? a[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int add = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int incl = s[threadIdx.x];
+    const int c = carry;
+    if (i < n) a[i] = c + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = c + incl;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scatter_order(const int *tdist, int n, int *cursor, int *order) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) order[atomicAdd(&cursor[tdist[t]], 1)] = t;
+}
+
+void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
+                       int *jf0, int *jf1, int *tdist, int *hist, int nbins,
+                       int *order, int max_tile_dim, int *max_dist,
+                       hipStream_t st) {
+  const int n = T.ntiles;
+  const int g = (n + 255) / 256;
+  const int d3 = shape_is_3d(shape) ? 1 : 0;
+  k_jf_init<<<g, 256, 0, st>>>(T, seed, jf0);
+  int *a = jf0, *b = jf1;
+  int top = 1;
+  while (top < max_tile_dim) top <<= 1;
+  for (int step = top >> 1; step >= 1; step >>= 1)
+    for (int rep = 0; rep < 2; ++rep) {
+      k_jf_pass<<<g, 256, 0, st>>>(T, a, b, step, d3);
+      int *tmp = a; a = b; b = tmp;
+    }
+  k_jf_dist<<<g, 256, 0, st>>>(T, a, tdist, nbins);
+  if (max_dist) {
+    launch_fill_i32(max_dist, 1, 0, st);
+    k_max_dist<<<g, 256, 0, st>>>(tdist, n, nbins - 1, max_dist);
+  }
+  launch_fill_i32(hist, nbins, 0, st);
+  k_hist<<<g, 256, 0, st>>>(tdist, n, hist);
+  k_scan_1wg<<<1, 1024, 0, st>>>(hist, nbins);
+  k_scatter_order<<<g, 256, 0, st>>>(tdist, n, hist, order);
+}
+
+// ---- list compaction (ballot + one atomic per wave) ------------------------------
+
+__global__ void k_compact(const int *src, int n, int reverse, unsigned char *flag,
+                          Queue q, int *sweep_cnt, long long *total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = -1;
+  bool f = false;
+  if (i < n) {
+    t = src ? src[reverse ? n - 1 - i : i] : (reverse ? n - 1 - i : i);
+    f = flag[t] != 0;
+    if (f) flag[t] = 0;
+  }
+  wave_append(f, t, q.list, q.ctr + 2 * q.slot, sweep_cnt, total);
+}
+
+void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
+                    const Queue &q, int *sweep_cnt, long long *total,
+                    hipStream_t st) {
+  if (n <= 0) return;
+  k_compact<<<(n + 255) / 256, 256, 0, st>>>(src, n, reverse, flag, q, sweep_cnt, total);
+}
+
+// ---- persistent tile queue ----------------------------------------------------------
+
+struct QueueCursor {
+  int count;
+  int *head;
+  __device__ __forceinline__ explicit QueueCursor(const Queue &q) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      q.ctr[2 * (q.slot ^ 1)] = 0;
+      q.ctr[2 * (q.slot ^ 1) + 1] = 0;
+    }
+    count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[2 * q.slot];
+    head = q.ctr + 2 * q.slot + 1;
+  }
+};
+
+// ---- block-wise push-relabel -------------------------------------------------------
+//
+// One workgroup discharges one tile at a time: e, h and the K residual planes
+// of the tile go to LDS with a halo of neighbour heights.  Each local
+// iteration relabels every active node (e > 0, finite h) to
+// min over residual arcs of h_nb + 1 when it has no admissible arc, then
+// pushes min(e, c) along arcs with h == h_nb + 1 in direction order.  Heights
+// are not written during the push phase, so an arc is never pushed both ways
+// at once and the reverse residual of an arc is touched by one lane only;
+// only excess needs LDS atomics.  Flow pushed into halo cells accumulates in
+// their LDS slots and is flushed to HBM once per visit.  The loop ends when a
+// whole iteration pushes nothing (graphcut_push_relabel.cl:77-182) or after
+// max_iters iterations.
+template <class S>
+__global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
+                                              unsigned char *flag, int max_iters) {
+  __shared__ int s_h[S::PADN];
+  __shared__ int s_e[S::PADN];
+  __shared__ int s_c[S::KMAX][S::PADN];
+  __shared__ int s_idx;
+  const int tid = threadIdx.x;
+  const Lane<S> L(tid);
+  const QueueCursor Q(q);
+  const long long st = P.stride;
+  for (;;) {
+    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= Q.count) break;
+    const int t = q.list[idx];
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int K = I.K;
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
+    long long gv = 0;
+    int e0 = 0, h0 = HINF;
+    int c0r[S::KMAX];
+    if (valid) {
+      gv = node_index(I, gx, gy, gz);
+      e0 = P.e[gv];
+      h0 = P.h[gv];
+    }
+#pragma unroll
+    for (int d = 0; d < S::KMAX; ++d)
+      c0r[d] = (valid && d < K) ? P.cap[d * st + gv] : 0;
+    s_e[L.me] = e0;
+    s_h[L.me] = h0;
+#pragma unroll
+    for (int d = 0; d < S::KMAX; ++d) s_c[d][L.me] = c0r[d];
+    for (int p = tid; p < S::PADN; p += S::NT) {
+      int x, y, z, nout;
+      bool inter;
+      pad_coords<S>(p, x, y, z, inter, nout);
+      if (inter) continue;
+      int hv = HINF;
+      if (halo_needed<S>(nout, K)) {
+        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
+          hv = P.h[node_index(I, X, Y, Z)];
+      }
+      s_h[p] = hv;
+      s_e[p] = 0;
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) s_c[d][p] = 0;
+    }
+    __syncthreads();
+
+    int it = 0, any;
+    do {
+      // relabel (monotone; neighbours may be read before or after their own
+      // relabel in this iteration, which keeps the labelling valid)
+      if (valid) {
+        const int e = s_e[L.me];
+        const int h = s_h[L.me];
+        if (e > 0 && h < HINF) {
+          int m = HINF;
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d)
+            if (d < K && s_c[d][L.me] > 0) m = min(m, s_h[L.me + lds_off<S>(d)] + 1);
+          if (m > h) s_h[L.me] = m;
+        }
+      }
+      __syncthreads();
+      // push
+      int pushed = 0;
+      if (valid) {
+        const int e = s_e[L.me];
+        const int h = s_h[L.me];
+        if (e > 0 && h < HINF) {
+          int rem = e;
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d) {
+            if (d < K && rem > 0) {
+              const int nb = L.me + lds_off<S>(d);
+              const int c = s_c[d][L.me];
+              if (c > 0 && h == s_h[nb] + 1) {
+                const int f = min(rem, c);
+                s_c[d][L.me] = c - f;
+                s_c[d ^ 1][nb] += f;
+                atomicAdd(&s_e[nb], f);
+                rem -= f;
+              }
+            }
+          }
+          if (rem != e) {
+            atomicAdd(&s_e[L.me], rem - e);
+            pushed = 1;
+          }
+        }
+      }
+      any = __syncthreads_or(pushed);
+    } while (any && ++it < max_iters);
+
+    // write back what changed
+    int act = 0;
+    if (valid) {
+      const int e1 = s_e[L.me], h1 = s_h[L.me];
+      if (e1 != e0) P.e[gv] = e1;
+      if (h1 != h0) P.h[gv] = h1;
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        if (d < K && s_c[d][L.me] != c0r[d]) P.cap[d * st + gv] = s_c[d][L.me];
+      act = e1 > 0 && h1 < HINF;
+    }
+    // flush flow that crossed into the halo: excess and the reverse residual
+    // of each crossing arc (graphcut_push_relabel.cl:184-195)
+    for (int p = tid; p < S::PADN; p += S::NT) {
+      int x, y, z, nout;
+      bool inter;
+      pad_coords<S>(p, x, y, z, inter, nout);
+      if (inter || !halo_needed<S>(nout, K)) continue;
+      const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+      if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) continue;
+      const long long g = node_index(I, X, Y, Z);
+      const int f = s_e[p];
+      if (f != 0) {
+        atomicAdd(&P.e[g], f);
+        flag[tile_of<S>(I, X, Y, Z)] = 1;
+      }
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        if (d < K) {
+          const int dc = s_c[d][p];
+          if (dc != 0) P.cap[d * st + g] += dc;
+        }
+    }
+    if (__syncthreads_or(act) && tid == 0) flag[t] = 1;
+  }
+}
+
+void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
+                 const Queue &q, unsigned char *flag, int max_iters,
+                 hipStream_t st) {
+  JFC_DISPATCH(shape, (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, max_iters)));
+}
+
+// ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
+//
+// h(u) = min(h(u), min over arcs with residual > 0 of h(v) + 1), iterated in
+// LDS until the tile is stable; tiles whose halo changed are revisited.  The
+// fixpoint over all tiles is the exact breadth-first distance to the nearest
+// deficit node (graphcut_global_relabel.cl:50-124).
+template <class S>
+__global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
+                                               unsigned char *dirty) {
+  __shared__ int s_h[S::PADN];
+  __shared__ int s_idx;
+  const int tid = threadIdx.x;
+  const Lane<S> L(tid);
+  const QueueCursor Q(q);
+  const long long st = P.stride;
+  for (;;) {
+    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= Q.count) break;
+    const int t = q.list[idx];
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int K = I.K;
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
+    long long gv = 0;
+    int h0 = HINF;
+    unsigned mask = 0;
+    if (valid) {
+      gv = node_index(I, gx, gy, gz);
+      h0 = P.h[gv];
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        if (d < K && P.cap[d * st + gv] > 0) mask |= 1u << d;
+    }
+    s_h[L.me] = h0;
+    for (int p = tid; p < S::PADN; p += S::NT) {
+      int x, y, z, nout;
+      bool inter;
+      pad_coords<S>(p, x, y, z, inter, nout);
+      if (inter) continue;
+      int hv = HINF;
+      if (halo_needed<S>(nout, K)) {
+        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
+          hv = P.h[node_index(I, X, Y, Z)];
+      }
+      s_h[p] = hv;
+    }
+    __syncthreads();
+    int any;
+    do {
+      int ch = 0;
+      if (mask) {
+        const int h = s_h[L.me];
+        int m = h;
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d)
+          if (mask & (1u << d)) m = min(m, s_h[L.me + lds_off<S>(d)] + 1);
+        if (m < h) {
+          s_h[L.me] = m;
+          ch = 1;
+        }
+      }
+      any = __syncthreads_or(ch);
+    } while (any);
+    if (valid) {
+      const int h1 = s_h[L.me];
+      if (h1 != h0) {
+        P.h[gv] = h1;
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) {
+          if (d >= K) break;
+          int dx, dy, dz;
+          dir_of<S::D3>(d, dx, dy, dz);
+          const int lx = L.lx + dx, ly = L.ly + dy, lz = L.lz + dz;
+          if (lx >= 0 && lx < S::TW && ly >= 0 && ly < S::TH && lz >= 0 && lz < S::TD) continue;
+          const int X = gx + dx, Y = gy + dy, Z = gz + dz;
+          if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) continue;
+          dirty[tile_of<S>(I, X, Y, Z)] = 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
+                  const Queue &q, unsigned char *dirty, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_relax<S><<<nwg, 256, 0, st>>>(T, P, q, dirty)));
+}
+
+// ---- canonical cut: residual reachability from excess nodes -------------------------
+//
+// A labelled node labels every neighbour it has residual capacity toward
+// (graphcut_compute_cut.cl:95-113, here from the tail of the arc so only the
+// node's own planes are read).  Labels entering the halo are written to HBM
+// and mark the neighbour tile for the next pass.  A labelled node with a
+// deficit means an augmenting path still exists: *cert is set
+// (graphcut_compute_cut.cl:135-139).
+template <class S>
+__global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
+                                               unsigned char *dirty, int *cert) {
+  __shared__ unsigned char s_l[S::PADN];
+  __shared__ int s_idx;
+  const int tid = threadIdx.x;
+  const Lane<S> L(tid);
+  const QueueCursor Q(q);
+  const long long st = P.stride;
+  for (;;) {
+    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= Q.count) break;
+    const int t = q.list[idx];
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int K = I.K;
+    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+    const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
+    long long gv = 0;
+    unsigned char l0 = 0;
+    unsigned mask = 0;
+    bool deficit = false;
+    if (valid) {
+      gv = node_index(I, gx, gy, gz);
+      l0 = P.lab[gv];
+      deficit = P.e[gv] < 0;
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        if (d < K && P.cap[d * st + gv] > 0) mask |= 1u << d;
+      if (l0 && deficit) *cert = 1;
+    }
+    s_l[L.me] = l0;
+    for (int p = tid; p < S::PADN; p += S::NT) {
+      int x, y, z, nout;
+      bool inter;
+      pad_coords<S>(p, x, y, z, inter, nout);
+      if (inter) continue;
+      unsigned char lv = 1;
+      if (halo_needed<S>(nout, K)) {
+        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
+          lv = P.lab[node_index(I, X, Y, Z)];
+      }
+      s_l[p] = lv;
+    }
+    __syncthreads();
+    int any;
+    do {
+      int ch = 0;
+      if (mask && s_l[L.me]) {
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d)
+          if (mask & (1u << d)) {
+            const int nb = L.me + lds_off<S>(d);
+            if (s_l[nb] == 0) {
+              s_l[nb] = 2;
+              ch = 1;
+            }
+          }
+      }
+      any = __syncthreads_or(ch);
+    } while (any);
+    if (valid && s_l[L.me] == 2) {
+      P.lab[gv] = 1;
+      if (deficit) *cert = 1;
+    }
+    for (int p = tid; p < S::PADN; p += S::NT) {
+      if (s_l[p] != 2) continue;
+      int x, y, z, nout;
+      bool inter;
+      pad_coords<S>(p, x, y, z, inter, nout);
+      if (inter) continue;
+      const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+      P.lab[node_index(I, X, Y, Z)] = 1;
+      dirty[tile_of<S>(I, X, Y, Z)] = 1;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
+                  const Queue &q, unsigned char *dirty, int *cert,
+                  hipStream_t st) {
+  JFC_DISPATCH(shape, (k_reach<S><<<nwg, 256, 0, st>>>(T, P, q, dirty, cert)));
+}
+
+}  // namespace jfc

@@ -1,0 +1,410 @@
+"""ctypes binding of libjfcut (include/jfcut.h) and a torch.autograd.Function.
+
+This is the project's only Python: a thin shim over the C-ABI.  Host inputs are
+numpy arrays (node order x + nx*(y + ny*z), i.e. C-contiguous arrays of shape
+(ny, nx) or (nz, ny, nx)); device inputs are torch tensors on the GPU, passed
+by pointer.  Nothing here computes: if libjfcut.so is missing the import of
+the library fails loudly.
+
+Reference surface (src/QVisualizer/Graph Cut/QGCWidget.cpp): loading a graph
+(slotLoadGraphClicked :2631-2722), modeling (slotModelClicked :2560-2606),
+cutting (slotCutClicked :2364-2558), reading labels and energy (:2452-2520,
+:2990-2996).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjfcut.so")
+
+OK, NOT_CONVERGED = 0, 1
+ERR_INVALID, ERR_DEVICE, ERR_OOM, ERR_STATE, ERR_OVERFLOW, ERR_NO_SEEDS = -1, -2, -3, -4, -5, -6
+MAX_TERMINAL_CAP = 1 << 30
+MAX_NEIGHBOR_CAP = 1 << 26
+TILE = {"default": 0, "32x8": 1, "64x4": 2, "16x16": 3, "8x8x4": 4, "32x4x2": 5}
+
+# direction d -> (dx, dy, dz); reverse of d is d ^ 1
+DIRS_2D = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0),
+           (1, 1, 0), (-1, -1, 0), (-1, 1, 0), (1, -1, 0)]
+DIRS_3D = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("max_sweeps", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("relabel_work", ctypes.c_double), ("max_local_iters", ctypes.c_int),
+                ("tile_shape", ctypes.c_int), ("profile", ctypes.c_int),
+                ("stream", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("flow", ctypes.c_int64), ("converged", ctypes.c_int),
+                ("sweeps", ctypes.c_int), ("global_relabels", ctypes.c_int),
+                ("relabel_passes", ctypes.c_int), ("cut_passes", ctypes.c_int),
+                ("certificate_failures", ctypes.c_int),
+                ("push_tiles", ctypes.c_int64), ("push_launches", ctypes.c_int64),
+                ("relabel_tiles", ctypes.c_int64), ("cut_tiles", ctypes.c_int64),
+                ("push_bytes", ctypes.c_int64), ("ms_total", ctypes.c_double),
+                ("ms_push", ctypes.c_double), ("ms_relabel", ctypes.c_double),
+                ("ms_cut", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ModelParams(ctypes.Structure):
+    _fields_ = [("k_fg", ctypes.c_int), ("k_bg", ctypes.c_int), ("kmeans_iters", ctypes.c_int),
+                ("unary_scale", ctypes.c_int64), ("range", ctypes.c_int64),
+                ("lambda_xy", ctypes.c_double), ("lambda_z", ctypes.c_double),
+                ("z_separate", ctypes.c_int)]
+
+
+class ModelStats(ctypes.Structure):
+    _fields_ = [("iters_fg", ctypes.c_int), ("iters_bg", ctypes.c_int),
+                ("centroids_fg", ctypes.c_int64 * 48), ("centroids_bg", ctypes.c_int64 * 48),
+                ("seeds_fg", ctypes.c_int64), ("seeds_bg", ctypes.c_int64),
+                ("pair_sum", ctypes.c_int64 * 2), ("pair_count", ctypes.c_int64 * 2)]
+
+
+class JFCError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libjfcut error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy",
+           "jfc_num_instances", "jfc_last_error", "jfc_set_terminal_caps", "jfc_set_excess",
+           "jfc_set_neighbor_caps", "jfc_set_seeds", "jfc_solve", "jfc_get_flow",
+           "jfc_get_flows", "jfc_get_labels", "jfc_get_energy", "jfc_backward",
+           "jfc_model_image", "jfc_synth_image", "jfc_synth_video", "jfc_synth_volume",
+           "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
+           "jfc_debug_tile_order"]
+
+
+def lib():
+    """Load libjfcut.so (built in place by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    P, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    sig = {
+        "jfc_default_options": (None, [ctypes.POINTER(Options)]),
+        "jfc_create": (I, [ctypes.POINTER(P), I, I, I, I, I]),
+        "jfc_create_batch": (I, [ctypes.POINTER(P), I, P]),
+        "jfc_destroy": (None, [P]),
+        "jfc_num_instances": (I, [P]),
+        "jfc_last_error": (ctypes.c_char_p, []),
+        "jfc_set_terminal_caps": (I, [P, I, P, P, I]),
+        "jfc_set_excess": (I, [P, I, P, I]),
+        "jfc_set_neighbor_caps": (I, [P, I, I, P, I]),
+        "jfc_set_seeds": (I, [P, I, P, P, I]),
+        "jfc_solve": (I, [P, ctypes.POINTER(Options), ctypes.POINTER(Stats)]),
+        "jfc_get_flow": (I, [P, I, ctypes.POINTER(I64)]),
+        "jfc_get_flows": (I, [P, P]),
+        "jfc_get_labels": (I, [P, I, P, I]),
+        "jfc_get_energy": (I, [P, I, ctypes.POINTER(I64)]),
+        "jfc_backward": (I, [P, I, P, P, P, I]),
+        "jfc_model_image": (I, [P, I, P, I, I, P, I, ctypes.POINTER(ModelParams),
+                                ctypes.POINTER(ModelStats)]),
+        "jfc_synth_image": (I, [P, P, I, I, I, ctypes.c_double, ctypes.c_uint64]),
+        "jfc_synth_video": (I, [P, P, I, I, I, I, ctypes.c_double, ctypes.c_uint64]),
+        "jfc_synth_volume": (I, [P, P, I, I, I, I, ctypes.c_uint64]),
+        "jfc_get_terminal_caps": (I, [P, I, P, P, I]),
+        "jfc_get_neighbor_caps": (I, [P, I, I, P, I]),
+        "jfc_debug_tiles": (I, [P, I, P, P, P, P]),
+        "jfc_debug_tile_order": (I, [P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, allow=(OK,)):
+    if rc not in allow:
+        raise JFCError(rc, lib().jfc_last_error().decode())
+    return rc
+
+
+def _ptr(a, dtype):
+    """(pointer, on_device) of a numpy array or a GPU torch tensor."""
+    if a is None:
+        return None, 0
+    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+        return ctypes.c_void_p(a.data_ptr()), 1
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr, 0
+
+
+def _host_or_dev(a, dtype):
+    p, dev = _ptr(a, dtype)
+    if p is None:
+        return None, 0, None
+    if dev:
+        return p, 1, a
+    return p.ctypes.data_as(ctypes.c_void_p), 0, p
+
+
+def default_options(**kw):
+    o = Options()
+    lib().jfc_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "tile_shape" and isinstance(v, str):
+            v = TILE[v]
+        setattr(o, k, v)
+    return o
+
+
+class GridGraph:
+    """One or more grid instances solved together (jfc_create / jfc_create_batch)."""
+
+    def __init__(self, shapes):
+        """shapes: list of (nx, ny, nz, connectivity); nz == 1 is 2D."""
+        self._h = ctypes.c_void_p()
+        arr = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 4))
+        self.shapes = [tuple(int(v) for v in row) for row in arr]
+        _check(lib().jfc_create_batch(ctypes.byref(self._h), len(arr),
+                                      arr.ctypes.data_as(ctypes.c_void_p)))
+
+    @classmethod
+    def grid(cls, nx, ny, nz=1, connectivity=4):
+        return cls([(nx, ny, nz, connectivity)])
+
+    def close(self):
+        if self._h:
+            lib().jfc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def nodes(self, inst=0):
+        nx, ny, nz, _ = self.shapes[inst]
+        return nx * ny * nz
+
+    def ndirs(self, inst=0):
+        return self.shapes[inst][3]
+
+    # ---- inputs
+    def set_terminal_caps(self, cs, ct, inst=0):
+        p1, d1, k1 = _host_or_dev(cs, np.int32)
+        p2, d2, k2 = _host_or_dev(ct, np.int32)
+        _check(lib().jfc_set_terminal_caps(self._h, inst, p1, p2, d1))
+
+    def set_excess(self, ef, inst=0):
+        p, d, k = _host_or_dev(ef, np.int32)
+        _check(lib().jfc_set_excess(self._h, inst, p, d))
+
+    def set_neighbor_caps(self, arc, cap, inst=0):
+        p, d, k = _host_or_dev(cap, np.int32)
+        _check(lib().jfc_set_neighbor_caps(self._h, inst, arc, p, d))
+
+    def set_all_neighbor_caps(self, caps, inst=0):
+        for d in range(self.ndirs(inst)):
+            self.set_neighbor_caps(d, caps[d], inst)
+
+    def set_seeds(self, fg=None, bg=None, inst=0):
+        p1, d1, k1 = _host_or_dev(fg, np.uint8)
+        p2, d2, k2 = _host_or_dev(bg, np.uint8)
+        _check(lib().jfc_set_seeds(self._h, inst, p1, p2, max(d1, d2)))
+
+    def model_image(self, image, seeds, inst=0, k_fg=5, k_bg=5, kmeans_iters=32,
+                    unary_scale=100, rng=255, lambda_xy=50.0, lambda_z=None, z_separate=0):
+        img_dtype = np.uint16 if getattr(image, "dtype", None) in (np.uint16,) else np.uint8
+        if hasattr(image, "element_size"):
+            bpc = image.element_size()
+        else:
+            bpc = np.dtype(img_dtype).itemsize
+        channels = 3 if (getattr(image, "ndim", 1) >= 3 and image.shape[-1] == 3) else 1
+        p1, d1, k1 = _host_or_dev(image, img_dtype)
+        p2, d2, k2 = _host_or_dev(seeds, np.uint8)
+        prm = ModelParams(k_fg, k_bg, kmeans_iters, unary_scale, rng, float(lambda_xy),
+                          float(lambda_xy if lambda_z is None else lambda_z), int(z_separate))
+        st = ModelStats()
+        _check(lib().jfc_model_image(self._h, inst, p1, channels, bpc, p2, d1,
+                                     ctypes.byref(prm), ctypes.byref(st)))
+        return st
+
+    # ---- solve
+    def solve(self, options=None, **kw):
+        o = options if options is not None else default_options(**kw)
+        st = Stats()
+        rc = _check(lib().jfc_solve(self._h, ctypes.byref(o), ctypes.byref(st)),
+                    allow=(OK, NOT_CONVERGED))
+        self.last_rc = rc
+        return st
+
+    # ---- outputs
+    def flow(self, inst=0):
+        f = ctypes.c_int64()
+        _check(lib().jfc_get_flow(self._h, inst, ctypes.byref(f)))
+        return f.value
+
+    def flows(self):
+        out = np.zeros(len(self.shapes), np.int64)
+        _check(lib().jfc_get_flows(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def labels(self, inst=0, out=None):
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(lib().jfc_get_labels(self._h, inst, ctypes.c_void_p(out.data_ptr()), 1))
+            return out
+        lab = np.zeros(self.nodes(inst), np.uint8)
+        _check(lib().jfc_get_labels(self._h, inst, lab.ctypes.data_as(ctypes.c_void_p), 0))
+        return lab
+
+    def terminal_caps(self, inst=0):
+        n = self.nodes(inst)
+        cs, ct = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().jfc_get_terminal_caps(self._h, inst, cs.ctypes.data_as(ctypes.c_void_p),
+                                           ct.ctypes.data_as(ctypes.c_void_p), 0))
+        return cs, ct
+
+    def neighbor_caps(self, inst=0):
+        n, K = self.nodes(inst), self.ndirs(inst)
+        out = np.zeros((K, n), np.int32)
+        for d in range(K):
+            _check(lib().jfc_get_neighbor_caps(self._h, inst, d,
+                                               out[d].ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def debug_tiles(self, inst=0):
+        nt = ctypes.c_int32()
+        td = (ctypes.c_int32 * 3)()
+        gd = (ctypes.c_int32 * 3)()
+        off = ctypes.c_int32()
+        _check(lib().jfc_debug_tiles(self._h, inst, ctypes.byref(nt), td, gd, ctypes.byref(off)))
+        return nt.value, tuple(td), tuple(gd), off.value
+
+    def debug_tile_order(self, tile_seed):
+        nt = self.debug_tiles(0)[0]
+        seed = np.ascontiguousarray(tile_seed, dtype=np.uint8)
+        assert seed.size == nt
+        dist = np.zeros(nt, np.int32)
+        order = np.zeros(nt, np.int32)
+        mx = ctypes.c_int32()
+        vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _check(lib().jfc_debug_tile_order(self._h, vp(seed), vp(dist), vp(order),
+                                          ctypes.byref(mx)))
+        return dist, order, mx.value
+
+    def energy(self, inst=0):
+        e = ctypes.c_int64()
+        _check(lib().jfc_get_energy(self._h, inst, ctypes.byref(e)))
+        return e.value
+
+    def backward(self, inst=0, out=None):
+        """(d_source, d_sink, d_neighbor[K, N]) as fp32 numpy arrays, or into the
+        given GPU tensors (d_source, d_sink, d_neighbor)."""
+        if out is not None:
+            a, b, c = out
+            ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+            _check(lib().jfc_backward(self._h, inst, ptr(a), ptr(b), ptr(c), 1))
+            return out
+        n, K = self.nodes(inst), self.ndirs(inst)
+        a = np.zeros(n, np.float32)
+        b = np.zeros(n, np.float32)
+        c = np.zeros((K, n), np.float32)
+        vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _check(lib().jfc_backward(self._h, inst, vp(a), vp(b), vp(c), 0))
+        return a, b, c
+
+
+# ---- synthetic workloads --------------------------------------------------------
+
+def synth_image(W, H, nblobs=8, noise=25.0, seed=0):
+    rgb = np.zeros((H, W, 3), np.uint8)
+    seeds = np.zeros((H, W), np.uint8)
+    _check(lib().jfc_synth_image(rgb.ctypes.data_as(ctypes.c_void_p),
+                                 seeds.ctypes.data_as(ctypes.c_void_p), W, H, nblobs,
+                                 float(noise), seed))
+    return rgb, seeds
+
+
+def synth_video(W, H, T, nblobs=8, noise=20.0, seed=0):
+    rgb = np.zeros((T, H, W, 3), np.uint8)
+    seeds = np.zeros((T, H, W), np.uint8)
+    _check(lib().jfc_synth_video(rgb.ctypes.data_as(ctypes.c_void_p),
+                                 seeds.ctypes.data_as(ctypes.c_void_p), W, H, T, nblobs,
+                                 float(noise), seed))
+    return rgb, seeds
+
+
+def synth_volume(nx, ny, nz, nell=12, seed=0):
+    vol = np.zeros((nz, ny, nx), np.uint16)
+    seeds = np.zeros((nz, ny, nx), np.uint8)
+    _check(lib().jfc_synth_volume(vol.ctypes.data_as(ctypes.c_void_p),
+                                  seeds.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, nell, seed))
+    return vol, seeds
+
+
+# ---- differentiable min cut -------------------------------------------------------
+
+def _autograd_function():
+    import torch
+
+    class MinCut(torch.autograd.Function):
+        """E(cs, ct, w) = min over cuts of sum_T cs + sum_S ct + sum_{S->T} w.
+
+        Inputs are GPU tensors holding integer values (any float dtype):
+        cs, ct of shape (N,), w of shape (K, N).  Forward returns the exact
+        min-cut energy (= max-flow) as a float64 scalar; backward returns the
+        exact subgradient from the canonical cut: 1[v in T], 1[v in S],
+        1[u in S and v in T] (jfc_backward)."""
+
+        @staticmethod
+        def forward(ctx, cs, ct, w, shape, connectivity):
+            nx, ny, nz = shape
+            g = GridGraph([(nx, ny, nz, connectivity)])
+            ics = cs.detach().round().to(torch.int32).contiguous()
+            ict = ct.detach().round().to(torch.int32).contiguous()
+            iw = w.detach().round().to(torch.int32).contiguous()
+            torch.cuda.synchronize()
+            g.set_terminal_caps(ics, ict)
+            for d in range(connectivity):
+                g.set_neighbor_caps(d, iw[d])
+            g.solve()
+            n = nx * ny * nz
+            dcs = torch.empty(n, dtype=torch.float32, device=cs.device)
+            dct = torch.empty_like(dcs)
+            dw = torch.empty((connectivity, n), dtype=torch.float32, device=cs.device)
+            g.backward(out=(dcs, dct, dw))
+            flow = g.flow()
+            g.close()
+            ctx.save_for_backward(dcs, dct, dw)
+            return torch.tensor(float(flow), dtype=torch.float64, device=cs.device)
+
+        @staticmethod
+        def backward(ctx, grad):
+            dcs, dct, dw = ctx.saved_tensors
+            gs = grad.to(torch.float32)
+            return gs * dcs, gs * dct, gs * dw, None, None
+
+    return MinCut
+
+
+_MINCUT = None
+
+
+def min_cut_energy(cs, ct, w, shape, connectivity):
+    """Differentiable exact min-cut energy (torch.autograd.Function)."""
+    global _MINCUT
+    if _MINCUT is None:
+        _MINCUT = _autograd_function()
+    return _MINCUT.apply(cs, ct, w, tuple(shape), connectivity)

@@ -1,0 +1,98 @@
+"""Writes the golden fixtures under tests/golden/ (run in the build container,
+where the JF-Cut reference tree is mounted at /root/reference).
+
+1. ref_<name>.npz for the reference's own graph fixtures
+   src/QVisualizer/data/{flower,person,sponge}/normalized_<name>.raw.{ef,c1,c2,cut}:
+   the graph (ef int32; c1/c2 ushort2 remapped to direction planes +x,-x,+y,-y),
+   the published max-flow (latex/sources/section-5-results.tex:31,36,41) and the
+   canonical labels: flower's from the .cut object channel (byte 2), person's and
+   sponge's from BK303 (the reference's CPU path, oracle/_ref) with free nodes on
+   the sink side.
+2. c1_goldens.json: config-1 instances (512x512, 4-connected) built from the
+   synthetic generator of libjfcut plus the oracle's integer data term, solved by
+   BK303: flow, |S|, sha256 of labels and of the capacity planes.
+
+Only data is written: inputs and expected outputs.
+"""
+import ctypes
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import oracle_lib as ol  # noqa: E402
+
+REF = "/root/reference/src/QVisualizer/data"
+PUBLISHED = {"flower": 1208089, "person": 880461, "sponge": 343591}
+
+
+def resolution(path):
+    for line in open(path, errors="replace"):
+        if line.startswith("Resolution:"):
+            x, y, z = [int(v) for v in line.split(":")[1].split()]
+            return x, y, z
+    raise ValueError(path)
+
+
+def ref_fixtures():
+    for name, flow in PUBLISHED.items():
+        pre = f"{REF}/{name}/normalized_{name}.raw"
+        nx, ny, nz = resolution(pre + ".dat")
+        ef = np.fromfile(pre + ".ef", dtype=np.int32)
+        c1 = np.fromfile(pre + ".c1", dtype=np.uint16).reshape(-1, 2)
+        c2 = np.fromfile(pre + ".c2", dtype=np.uint16).reshape(-1, 2)
+        # c1 = (+x, +y), c2 = (-y, -x)  (graphcut_push_relabel.cl:81-87)
+        caps = np.stack([c1[:, 0], c2[:, 1], c1[:, 1], c2[:, 0]]).astype(np.uint16)
+        cut = np.fromfile(pre + ".cut", dtype=np.int8).reshape(-1, 4)
+        cs = np.maximum(ef, 0).astype(np.int32)
+        ct = np.maximum(-ef, 0).astype(np.int32)
+        bk_flow, bk_lab = ol.bk303(nx, ny, 1, 4, cs, ct, caps.astype(np.int32))
+        assert bk_flow == flow, (name, bk_flow)
+        if name == "flower":
+            lab = (cut[:, 2] != 0).astype(np.uint8)
+            assert np.array_equal(lab, bk_lab)
+            source = "cut-object-channel"
+        else:
+            lab = bk_lab
+            source = "bk303"
+        assert np.abs(ef).max() < 32768
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), nx=nx, ny=ny,
+                            ef=ef.astype(np.int16),
+                            caps=caps, flow=np.int64(flow), labels=np.packbits(lab),
+                            nlabels=np.int64(lab.size), label_source=source)
+        print(name, nx, ny, flow, int(lab.sum()), source)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def c1_goldens(seeds=(0, 1, 2, 3)):
+    sys.path.insert(0, os.path.join(ROOT, "jf-cut_amd"))
+    import jfcut
+    out = []
+    for s in seeds:
+        rgb, sd = jfcut.synth_image(512, 512, nblobs=8, noise=25.0, seed=s)
+        cs, ct, caps, info = ol.model_image(rgb, sd, 4, k=5, lam=50.0)
+        flow, lab = ol.bk303(512, 512, 1, 4, cs, ct, caps)
+        out.append({"seed": s, "W": 512, "H": 512, "conn": 4, "flow": int(flow),
+                    "source_set": int(lab.sum()), "labels_sha256": sha(lab),
+                    "image_sha256": sha(rgb), "seeds_sha256": sha(sd),
+                    "caps_sha256": sha(np.concatenate([cs, ct, caps.ravel()])),
+                    "centroids_fg": info["cfg"].tolist(), "centroids_bg": info["cbg"].tolist()})
+        if s == seeds[0]:
+            np.savez_compressed(os.path.join(HERE, "c1_seed0_labels.npz"), labels=np.packbits(lab),
+                                n=np.int64(lab.size))
+        print("c1 seed", s, flow, int(lab.sum()))
+    with open(os.path.join(HERE, "c1_goldens.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    ref_fixtures()
+    c1_goldens()

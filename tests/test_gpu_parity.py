@@ -1,0 +1,230 @@
+"""GPU parity: the HIP solver (through the C-ABI) against the oracle, BK303 and
+the committed golden fixtures.  Bit-exact: int64 flow and canonical labels."""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from conftest import load_c1_goldens, load_ref_fixture, random_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def solve_grid(jf, nx, ny, nz, conn, cs, ct, caps, **opt):
+    g = jf.GridGraph([(nx, ny, nz, conn)])
+    g.set_terminal_caps(cs, ct)
+    for d in range(conn):
+        g.set_neighbor_caps(d, caps[d])
+    st = g.solve(**opt)
+    return g, st
+
+
+@pytest.mark.parametrize("name", ["flower", "person", "sponge"])
+def test_reference_fixtures_gpu(gpu, name):
+    nx, ny, ef, cs, ct, caps, flow, lab = load_ref_fixture(name)
+    say("case", name, nx, ny)
+    g = gpu.GridGraph([(nx, ny, 1, 4)])
+    g.set_excess(ef)
+    for d in range(4):
+        g.set_neighbor_caps(d, caps[d])
+    st = g.solve()
+    assert st.converged == 1
+    assert g.flow() == flow
+    assert np.array_equal(g.labels(), lab)
+    assert g.energy() == flow
+    g.close()
+
+
+@pytest.mark.parametrize("conn,shape", [(4, (37, 29, 1)), (8, (41, 23, 1)), (6, (13, 11, 9)),
+                                        (4, (300, 200, 1)), (8, (257, 190, 1)),
+                                        (6, (40, 33, 21))])
+def test_random_grids_gpu(gpu, conn, shape):
+    rng = np.random.default_rng(conn * 100 + shape[0])
+    nx, ny, nz = shape
+    for trial in range(3):
+        say("case", conn, shape, trial)
+        cs, ct, caps = random_grid(rng, nx, ny, nz, conn)
+        f, l, ok = ol.maxflow(nx, ny, nz, conn, cs, ct, caps)
+        g, st = solve_grid(gpu, nx, ny, nz, conn, cs, ct, caps)
+        assert st.converged == 1
+        assert g.flow() == f
+        assert np.array_equal(g.labels(), l)
+        g.close()
+
+
+@pytest.mark.parametrize("opts", [dict(tile_shape="64x4"), dict(tile_shape="16x16"),
+                                  dict(max_local_iters=1), dict(relabel_work=0.0),
+                                  dict(check_every=1, max_local_iters=1000)])
+def test_options_do_not_change_result(gpu, opts):
+    rng = np.random.default_rng(99)
+    nx, ny = 203, 147
+    for conn in (4, 8):
+        cs, ct, caps = random_grid(rng, nx, ny, 1, conn)
+        f, l, ok = ol.maxflow(nx, ny, 1, conn, cs, ct, caps)
+        say("case", opts, conn)
+        g, st = solve_grid(gpu, nx, ny, 1, conn, cs, ct, caps, **opts)
+        assert g.flow() == f
+        assert np.array_equal(g.labels(), l)
+        g.close()
+
+
+def test_3d_tile_shape_option(gpu):
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 70, 20, 9
+    cs, ct, caps = random_grid(rng, nx, ny, nz, 6)
+    f, l, ok = ol.maxflow(nx, ny, nz, 6, cs, ct, caps)
+    g, st = solve_grid(gpu, nx, ny, nz, 6, cs, ct, caps, tile_shape="32x4x2")
+    assert g.flow() == f and np.array_equal(g.labels(), l)
+
+
+def test_c1_goldens_gpu(gpu):
+    """Config 1 end to end on the device: data term (k-means, unary, pairwise)
+    bit-exact with the oracle, then flow and labels equal to the goldens."""
+    for gd in load_c1_goldens():
+        say("case c1 seed", gd["seed"])
+        rgb, sd = gpu.synth_image(gd["W"], gd["H"], nblobs=8, noise=25.0, seed=gd["seed"])
+        assert sha(rgb) == gd["image_sha256"]
+        g = gpu.GridGraph([(gd["W"], gd["H"], 1, 4)])
+        ms = g.model_image(rgb, sd, k_fg=5, k_bg=5, lambda_xy=50.0)
+        assert np.array_equal(np.array(ms.centroids_fg[:15]).reshape(5, 3), np.array(gd["centroids_fg"]))
+        assert np.array_equal(np.array(ms.centroids_bg[:15]).reshape(5, 3), np.array(gd["centroids_bg"]))
+        cs, ct = g.terminal_caps()
+        caps = g.neighbor_caps()
+        assert sha(np.concatenate([cs, ct, caps.ravel()])) == gd["caps_sha256"]
+        st = g.solve()
+        assert g.flow() == gd["flow"]
+        lab = g.labels()
+        assert sha(lab) == gd["labels_sha256"]
+        assert g.energy() == gd["flow"]
+        g.close()
+
+
+def test_batch_gpu(gpu):
+    rng = np.random.default_rng(12)
+    shapes, data = [], []
+    for i in range(9):
+        conn = 4 if i % 2 else 8
+        nx, ny = int(rng.integers(20, 180)), int(rng.integers(20, 160))
+        shapes.append((nx, ny, 1, conn))
+        data.append(random_grid(rng, nx, ny, 1, conn))
+    g = gpu.GridGraph(shapes)
+    for i, (cs, ct, caps) in enumerate(data):
+        g.set_terminal_caps(cs, ct, inst=i)
+        for d in range(shapes[i][3]):
+            g.set_neighbor_caps(d, caps[d], inst=i)
+    st = g.solve()
+    assert st.converged == 1
+    flows = g.flows()
+    for i, (cs, ct, caps) in enumerate(data):
+        nx, ny, _, conn = shapes[i]
+        f, l, ok = ol.maxflow(nx, ny, 1, conn, cs, ct, caps)
+        assert flows[i] == f, i
+        assert np.array_equal(g.labels(i), l), i
+    g.close()
+
+
+def test_backward_gpu(gpu):
+    rng = np.random.default_rng(21)
+    nx, ny, conn = 40, 30, 8
+    cs, ct, caps = random_grid(rng, nx, ny, 1, conn, seed_frac=0.0)
+    g, st = solve_grid(gpu, nx, ny, 1, conn, cs, ct, caps)
+    lab = g.labels()
+    dcs, dct, dnb = g.backward()
+    assert np.array_equal(dcs, (lab == 0).astype(np.float32))
+    assert np.array_equal(dct, (lab == 1).astype(np.float32))
+    E0 = g.flow()
+    # +-1 integer finite differences on random non-tie neighbour arcs
+    idx = np.arange(nx * ny).reshape(ny, nx)
+    checked = 0
+    for trial in range(60):
+        d = int(rng.integers(0, conn))
+        v = int(rng.integers(0, nx * ny))
+        if caps[d][v] < 1:
+            continue
+        y, x = divmod(v, nx)
+        dx, dy, _ = gpu.DIRS_2D[d]
+        if not (0 <= x + dx < nx and 0 <= y + dy < ny):
+            continue
+        c_up = caps.copy(); c_up[d][v] += 1
+        c_dn = caps.copy(); c_dn[d][v] -= 1
+        Eu, _, _ = ol.maxflow(nx, ny, 1, conn, cs, ct, c_up)
+        Ed, _, _ = ol.maxflow(nx, ny, 1, conn, cs, ct, c_dn)
+        if Eu - E0 != E0 - Ed:
+            continue  # tie: the energy is not differentiable at this arc
+        assert dnb[d][v] == Eu - E0
+        checked += 1
+    assert checked > 10
+
+
+def test_not_converged_status(gpu):
+    rng = np.random.default_rng(5)
+    nx, ny = 300, 300
+    cs, ct, caps = random_grid(rng, nx, ny, 1, 4)
+    g = gpu.GridGraph([(nx, ny, 1, 4)])
+    g.set_terminal_caps(cs, ct)
+    for d in range(4):
+        g.set_neighbor_caps(d, caps[d])
+    st = g.solve(max_sweeps=1, check_every=1, max_local_iters=1, relabel_work=0.0)
+    assert g.last_rc in (gpu.OK, gpu.NOT_CONVERGED)
+    if g.last_rc == gpu.NOT_CONVERGED:
+        assert st.converged == 0
+
+
+def test_input_validation_gpu(gpu):
+    g = gpu.GridGraph([(16, 16, 1, 4)])
+    bad = np.full(256, gpu.MAX_TERMINAL_CAP + 1, np.int32)
+    with pytest.raises(gpu.JFCError) as e:
+        g.set_terminal_caps(bad, np.zeros(256, np.int32))
+    assert e.value.code == gpu.ERR_OVERFLOW
+    with pytest.raises(gpu.JFCError) as e:
+        g.set_neighbor_caps(0, np.full(256, gpu.MAX_NEIGHBOR_CAP + 1, np.int32))
+    assert e.value.code == gpu.ERR_OVERFLOW
+    with pytest.raises(gpu.JFCError) as e:
+        g.set_neighbor_caps(0, -np.ones(256, np.int32))
+    assert e.value.code == gpu.ERR_OVERFLOW
+    with pytest.raises(gpu.JFCError) as e:
+        g.solve()  # no terminal capacity anywhere: the reference's "no labels"
+    assert e.value.code == gpu.ERR_NO_SEEDS
+    with pytest.raises(gpu.JFCError):
+        g.flow()
+    g.close()
+
+
+def test_tile_order_gpu(gpu):
+    """Jump flooding over the tile grid = brute-force taxicab distance to the
+    nearest seed tile; the order is sorted by it; max distance = reduce."""
+    rng = np.random.default_rng(8)
+    for shape in [(640, 360, 1, 4), (1000, 70, 1, 8), (96, 80, 60, 6)]:
+        g = gpu.GridGraph([shape, shape])
+        nt, td, gd, off = g.debug_tiles(0)
+        _, _, gd1, off1 = g.debug_tiles(1)
+        seeds = (rng.random(nt) < 0.01).astype(np.uint8)
+        seeds[off1:] = 0
+        seeds[5] = 1
+        seeds[off1 + 3] = 1
+        dist, order, mx = g.debug_tile_order(seeds)
+        for inst, o in ((0, off), (1, off1)):
+            tx, ty, tz = gd
+            n = tx * ty * tz
+            zz, yy, xx = np.meshgrid(np.arange(tz), np.arange(ty), np.arange(tx), indexing="ij")
+            coords = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)
+            s = np.nonzero(seeds[o:o + n])[0]
+            if s.size == 0:
+                continue
+            bd = np.abs(coords[:, None, :] - coords[s][None, :, :]).sum(-1).min(1)
+            assert np.array_equal(dist[o:o + n], bd)
+        assert sorted(order.tolist()) == list(range(nt))
+        assert np.all(np.diff(dist[order]) >= 0)
+        assert mx == int(dist.max())  # every instance has a seed tile
+        g.close()
