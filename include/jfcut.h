@@ -71,6 +71,11 @@ typedef struct {
   int tile_shape;         /* JFC_TILE_* (0 = default for the dimensionality) */
   int profile;            /* 1 = time every push-relabel launch with HIP events */
   void *stream;           /* hipStream_t to issue work on (NULL = default stream) */
+  int initial_relabel;    /* 1 = exact global relabel before the first sweep; 0 =
+                             start from zero heights (the reference's default
+                             "no global relabel", QGCW:2394-2401) */
+  double stall_ratio;     /* run a global relabel when a sweep visits at least this
+                             fraction of the previous sweep's tiles (0 = off) */
 } jfc_options;
 
 #define JFC_TILE_DEFAULT 0
@@ -97,6 +102,7 @@ typedef struct {
   double ms_push;           /* summed push-relabel launch time (profile = 1) */
   double ms_relabel;        /* global relabel phases */
   double ms_cut;            /* canonical cut + certificate */
+  int64_t local_iters;      /* LDS discharge iterations summed over tile visits */
 } jfc_stats;
 
 /* ---- lifetime ------------------------------------------------------------ */

@@ -31,6 +31,11 @@ int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims 
 int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int32_t *tile_dist,
                          int32_t *order, int32_t *max_dist);
 
+/* Active tiles visited by each push-relabel sweep of the last solve (all
+ * colour classes of the sweep together); returns the number of sweeps, writes
+ * at most n counts. */
+int jfc_debug_sweep_counts(jfc_graph *g, int32_t *counts, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

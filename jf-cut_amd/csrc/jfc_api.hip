@@ -105,6 +105,7 @@ struct jfc_graph {
   long long *d_red = nullptr, *d_total = nullptr;
   int *d_sweep = nullptr;
   int sweep_cap = 0;
+  int last_sweeps = 0;
   int *d_flagword = nullptr;  // [0] certificate, [1] validation
   int slot = 0;
   bool solved = false;
@@ -156,7 +157,9 @@ struct jfc_graph {
       maxsum = std::max(maxsum, I.tx + I.ty + I.tz);
     }
     ntiles = off;
-    nbins = maxsum + 2;
+    // distance bins of the tile ordering (the last bin holds tiles of
+    // instances without seed tiles); farther tiles share the last finite bin
+    nbins = std::min(maxsum + 2, 4096);
     // colour classes: tiles of one class share no node or halo cell.
     // 2D 4C and 3D 6C: checkerboard; 2D 8C: 2x2 pattern.
     ncolors = (dim == 2 && !all4) ? 4 : 2;
@@ -240,6 +243,11 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->tile_shape = JFC_TILE_DEFAULT;
   o->profile = 0;
   o->stream = nullptr;
+  o->initial_relabel = 0;
+  o->stall_ratio = 0.97;
+  o->check_every = 2;
+  o->max_local_iters = 8;
+  o->relabel_work = 2.0;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -516,7 +524,7 @@ struct Solver {
     Queue q{g->d_order, g->d_ctr, g->slot, g->ntiles};
     const int nwg = std::min(g->nwg, g->ntiles);
     if (cut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
-    else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, st);
+    else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
     advance();
     *passes += 1;
     *tiles += g->ntiles;
@@ -530,7 +538,7 @@ struct Solver {
       if (cnt == 0) break;
       const int w = std::min(g->nwg, cnt);
       if (cut) launch_reach(g->shape, w, T, P, qc, g->d_dirty, g->d_flagword, st);
-      else launch_relax(g->shape, w, T, P, qc, g->d_dirty, st);
+      else launch_relax(g->shape, w, T, P, qc, g->d_dirty, g->d_flag, st);
       advance();
       *passes += 1;
       *tiles += cnt;
@@ -545,8 +553,7 @@ struct Solver {
     long long tiles = 0;
     int rc = search(false, &s.relabel_passes, &tiles);
     if (rc) return rc;
-    s.relabel_tiles += tiles;
-    launch_count_active(g->shape, g->tiles(), g->planes(), g->d_flag, st);
+    s.relabel_tiles += tiles;  // k_relax left every tile's active flag
     gr_ev.push_back({a, tm.mark()});
     s.global_relabels++;
     return JFC_OK;
@@ -594,13 +601,21 @@ struct Solver {
           red[i * RED_SLOTS + RED_CONST] == 0)
         return fail(JFC_ERR_NO_SEEDS, "instance " + std::to_string(i) +
                                           " has no terminal capacity (no labels)");
-    int rc = global_relabel();
-    if (rc) return rc;
+    int rc;
+    if (o.initial_relabel) {
+      if ((rc = global_relabel())) return rc;
+    } else {
+      // zero heights are a valid labelling (QGCW:2400 clears h)
+      launch_fill_i32(g->h, g->Ntot, 0, st);
+      launch_count_active(g->shape, T, P, g->d_flag, st);
+    }
     long long total_at_gr = 0;
     int since_check = 0;
     int cert_rounds = 0;
     bool done = false;
     int sweep = 0;
+    int sweep_at_gr = 0;
+    std::vector<int> recent;
     while (!done) {
       bool active = true;
       while (sweep < o.max_sweeps) {
@@ -611,7 +626,7 @@ struct Solver {
                          g->d_sweep + sweep, g->d_total, st);
           hipEvent_t a = o.profile ? tm.mark() : nullptr;
           launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
-                      o.max_local_iters, st);
+                      o.max_local_iters, g->d_total + 3, st);
           if (o.profile) push_ev.push_back({a, tm.mark()});
           advance();
           s.push_launches++;
@@ -619,18 +634,28 @@ struct Solver {
         ++sweep;
         if (++since_check >= o.check_every || sweep == o.max_sweeps) {
           since_check = 0;
-          int cnt = 0;
+          // active-tile counts of the sweeps since the last global relabel
+          const int first = std::max(sweep_at_gr, sweep - 2);
+          recent.assign(sweep - first, 0);
           long long total = 0;
-          JFC_HIP(hipMemcpyAsync(&cnt, g->d_sweep + sweep - 1, sizeof(int),
+          JFC_HIP(hipMemcpyAsync(recent.data(), g->d_sweep + first, recent.size() * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
           JFC_HIP(hipMemcpyAsync(&total, g->d_total, sizeof(long long),
                                  hipMemcpyDeviceToHost, st));
           JFC_HIP(hipStreamSynchronize(st));
+          const int cnt = recent.back();
           if (cnt == 0) { active = false; break; }
-          if (o.relabel_work > 0 &&
-              (double)(total - total_at_gr) > o.relabel_work * g->ntiles) {
+          // relabel when the active set stops shrinking (nodes cut off from
+          // every deficit keep sloshing until their heights are recomputed),
+          // or when the work since the last relabel exceeds the budget
+          const bool stalled = o.stall_ratio > 0 && recent.size() >= 2 &&
+                               cnt >= o.stall_ratio * recent[recent.size() - 2];
+          const bool budget = o.relabel_work > 0 &&
+                              (double)(total - total_at_gr) > o.relabel_work * g->ntiles;
+          if (stalled || budget) {
             if ((rc = global_relabel())) return rc;
             total_at_gr = total;
+            sweep_at_gr = sweep;
           }
         }
       }
@@ -662,6 +687,8 @@ struct Solver {
     s.flow = g->flows[0];
     s.converged = done ? 1 : 0;
     s.push_tiles = totals[0];
+    s.local_iters = totals[3];
+    g->last_sweeps = sweep;
     s.relabel_tiles += 0;
     s.push_bytes = totals[0] * 256LL * push_bytes_per_node(g->Kg);
     s.ms_total = Timer::ms(t0, t1);
@@ -792,6 +819,13 @@ extern "C" int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int3
   if (order) JFC_HIP(hipMemcpy(order, g->d_order, g->ntiles * 4, hipMemcpyDeviceToHost));
   if (max_dist) JFC_HIP(hipMemcpy(max_dist, g->d_flagword + 2, 4, hipMemcpyDeviceToHost));
   return JFC_OK;
+}
+
+extern "C" int jfc_debug_sweep_counts(jfc_graph *g, int32_t *counts, int32_t n) {
+  if (!g || !counts) return fail(JFC_ERR_INVALID, "null argument");
+  const int m = std::min(n, g->last_sweeps);
+  if (m > 0) JFC_HIP(hipMemcpy(counts, g->d_sweep, m * 4, hipMemcpyDeviceToHost));
+  return g->last_sweeps;
 }
 
 extern "C" int jfc_backward(jfc_graph *g, int inst, float *d_cap_source, float *d_cap_sink,

@@ -98,9 +98,10 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     hipStream_t st);
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, int max_iters,
-                 hipStream_t st);
+                 long long *iter_total, hipStream_t st);
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
-                  const Queue &q, unsigned char *dirty, hipStream_t st);
+                  const Queue &q, unsigned char *dirty, unsigned char *active,
+                  hipStream_t st);
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st);

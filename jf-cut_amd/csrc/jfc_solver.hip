@@ -19,6 +19,7 @@
 // over one colour class needs no global atomics except for halo excess.
 #include "jfc_device.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace jfc {
@@ -152,6 +153,31 @@ struct Chunk {
 
 static int tile_grid(int ntiles) { return ntiles < 4096 ? (ntiles > 0 ? ntiles : 1) : 4096; }
 
+// Streaming tile kernels keep TU tiles in flight per lane (their loads are
+// issued before any use) so that enough bytes are outstanding to cover HBM
+// latency with one 256-lane group per tile.
+constexpr int TU = 4;
+
+// Node of lane L in tile t; inst = -1 past the end of the chunk.
+struct TileNode {
+  int inst;
+  bool valid;
+  long long v;
+};
+
+template <class S>
+__device__ __forceinline__ TileNode tile_node(const Tiles &T, int t, int t_end,
+                                              const Lane<S> &L) {
+  TileNode n{-1, false, 0};
+  if (t >= t_end) return n;
+  const TileRef r = tile_ref<S>(T, t);
+  n.inst = T.tile_inst[t];
+  const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+  n.valid = gx < r.I.nx && gy < r.I.ny && gz < r.I.nz;
+  n.v = node_index(r.I, gx, gy, gz);
+  return n;
+}
+
 __device__ __forceinline__ void fold_flush(long long *red, int inst, long long a,
                                            long long b, long long c, long long *buf) {
   const long long A = block_sum_ll(a, buf);
@@ -173,22 +199,30 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red)
   const Chunk C(T.ntiles);
   int cur = -1;
   long long a = 0, b = 0, c = 0;
-  for (int t = C.t0; t < C.t1; ++t) {
-    const int inst = T.tile_inst[t];
-    if (inst != cur) {
-      if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
-      cur = inst;
-      a = b = c = 0;
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    int s[TU], k[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      s[u] = nd[u].valid ? P.cs[nd[u].v] : 0;
+      k[u] = nd[u].valid ? P.ct[nd[u].v] : 0;
     }
-    const TileRef r = tile_ref<S>(T, t);
-    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
-    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
-      const long long v = node_index(r.I, gx, gy, gz);
-      const int s = P.cs[v], k = P.ct[v];
-      P.e[v] = s - k;
-      a += s < k ? s : k;
-      b += k > s ? k - s : 0;
-      c += s > k ? s - k : 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      if (nd[u].inst < 0) break;
+      if (nd[u].inst != cur) {
+        if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
+        cur = nd[u].inst;
+        a = b = c = 0;
+      }
+      if (nd[u].valid) {
+        P.e[nd[u].v] = s[u] - k[u];
+        a += min(s[u], k[u]);
+        b += max(k[u] - s[u], 0);
+        c += max(s[u] - k[u], 0);
+      }
     }
   }
   if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
@@ -207,22 +241,26 @@ __global__ __launch_bounds__(256) void k_deficit(Tiles T, Planes P, long long *r
   const Chunk C(T.ntiles);
   int cur = -1;
   long long a = 0;
-  for (int t = C.t0; t < C.t1; ++t) {
-    const int inst = T.tile_inst[t];
-    if (inst != cur) {
-      if (cur >= 0) {
-        const long long A = block_sum_ll(a, buf);
-        if (threadIdx.x == 0)
-          atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    int e[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      if (nd[u].inst < 0) break;
+      if (nd[u].inst != cur) {
+        if (cur >= 0) {
+          const long long A = block_sum_ll(a, buf);
+          if (threadIdx.x == 0)
+            atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
+        }
+        cur = nd[u].inst;
+        a = 0;
       }
-      cur = inst;
-      a = 0;
-    }
-    const TileRef r = tile_ref<S>(T, t);
-    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
-    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
-      const int e = P.e[node_index(r.I, gx, gy, gz)];
-      a += e < 0 ? -e : 0;
+      a += max(-e[u], 0);
     }
   }
   if (cur >= 0) {
@@ -338,17 +376,23 @@ template <class S>
 __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
-  for (int t = C.t0; t < C.t1; ++t) {
-    const TileRef r = tile_ref<S>(T, t);
-    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
-    int s = 0;
-    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
-      const long long v = node_index(r.I, gx, gy, gz);
-      s = P.e[v] < 0;
-      P.h[v] = s ? 0 : HINF;
-    }
-    s = __syncthreads_or(s);
-    if (threadIdx.x == 0) seed[t] = (unsigned char)s;
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    int e[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
+    int bits = 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u)
+      if (nd[u].valid) {
+        P.h[nd[u].v] = e[u] < 0 ? 0 : HINF;
+        bits |= (e[u] < 0) << u;
+      }
+    bits = __syncthreads_or(bits);
+    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
+      seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
 }
 
@@ -363,17 +407,23 @@ template <class S>
 __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned char *seed) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
-  for (int t = C.t0; t < C.t1; ++t) {
-    const TileRef r = tile_ref<S>(T, t);
-    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
-    int s = 0;
-    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
-      const long long v = node_index(r.I, gx, gy, gz);
-      s = P.e[v] > 0;
-      P.lab[v] = (unsigned char)s;
-    }
-    s = __syncthreads_or(s);
-    if (threadIdx.x == 0) seed[t] = (unsigned char)s;
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    int e[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
+    int bits = 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u)
+      if (nd[u].valid) {
+        P.lab[nd[u].v] = (unsigned char)(e[u] > 0);
+        bits |= (e[u] > 0) << u;
+      }
+    bits = __syncthreads_or(bits);
+    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
+      seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
 }
 
@@ -389,16 +439,22 @@ __global__ __launch_bounds__(256) void k_count_active(Tiles T, Planes P,
                                                       unsigned char *flag) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
-  for (int t = C.t0; t < C.t1; ++t) {
-    const TileRef r = tile_ref<S>(T, t);
-    const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
-    int a = 0;
-    if (gx < r.I.nx && gy < r.I.ny && gz < r.I.nz) {
-      const long long v = node_index(r.I, gx, gy, gz);
-      a = P.e[v] > 0 && P.h[v] < HINF;
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    int e[TU], h[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
+      h[u] = nd[u].valid ? P.h[nd[u].v] : HINF;
     }
-    a = __syncthreads_or(a);
-    if (threadIdx.x == 0) flag[t] = (unsigned char)a;
+    int bits = 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u) bits |= (e[u] > 0 && h[u] < HINF) << u;
+    bits = __syncthreads_or(bits);
+    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
+      flag[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
 }
 
@@ -479,9 +535,19 @@ __global__ void k_max_dist(const int *tdist, int n, int unreachable, int *out) {
   if (lane_id() == 0 && d > 0) atomicMax(out, d);
 }
 
-__global__ void k_hist(const int *tdist, int n, int *hist) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) atomicAdd(&hist[tdist[t]], 1);
+// Histogram of tile distances (graphcut_bfs_histogram): counted in LDS per
+// workgroup, then one global atomic per non-empty bin -- most tiles share a
+// few small distances, so direct global atomics would serialise on them.
+__global__ __launch_bounds__(256) void k_hist(const int *tdist, int n, int nbins, int *hist) {
+  extern __shared__ int sh[];
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) sh[i] = 0;
+  __syncthreads();
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int t0 = blockIdx.x * per, t1 = min(n, t0 + per);
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) atomicAdd(&sh[tdist[t]], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+    if (sh[i]) atomicAdd(&hist[i], sh[i]);
 }
 
 // In-place exclusive scan of n ints by one 1024-lane workgroup (the clogs
@@ -511,9 +577,28 @@ __global__ __launch_bounds__(1024) void k_scan_1wg(int *a, int n) {
   }
 }
 
-__global__ void k_scatter_order(const int *tdist, int n, int *cursor, int *order) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) order[atomicAdd(&cursor[tdist[t]], 1)] = t;
+// Scatter tiles to their sorted position (graphcut_bfs_mapping): each
+// workgroup reserves one run per non-empty bin, then places its tiles inside
+// the runs by LDS rank.
+__global__ __launch_bounds__(256) void k_scatter_order(const int *tdist, int n, int nbins,
+                                                       int *cursor, int *order) {
+  extern __shared__ int sh[];
+  int *cnt = sh, *base = sh + nbins;
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int t0 = blockIdx.x * per, t1 = min(n, t0 + per);
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) atomicAdd(&cnt[tdist[t]], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+    base[i] = cnt[i] ? atomicAdd(&cursor[i], cnt[i]) : 0;
+    cnt[i] = 0;
+  }
+  __syncthreads();
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const int d = tdist[t];
+    order[base[d] + atomicAdd(&cnt[d], 1)] = t;
+  }
 }
 
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
@@ -538,9 +623,10 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
     k_max_dist<<<g, 256, 0, st>>>(tdist, n, nbins - 1, max_dist);
   }
   launch_fill_i32(hist, nbins, 0, st);
-  k_hist<<<g, 256, 0, st>>>(tdist, n, hist);
+  const int gh = std::min(g, 512);
+  k_hist<<<gh, 256, nbins * sizeof(int), st>>>(tdist, n, nbins, hist);
   k_scan_1wg<<<1, 1024, 0, st>>>(hist, nbins);
-  k_scatter_order<<<g, 256, 0, st>>>(tdist, n, hist, order);
+  k_scatter_order<<<gh, 256, 2 * nbins * sizeof(int), st>>>(tdist, n, nbins, hist, order);
 }
 
 // ---- list compaction (ballot + one atomic per wave) ------------------------------
@@ -567,8 +653,12 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 
 // ---- persistent tile queue ----------------------------------------------------------
 
+// Workgroups reserve runs of `chunk` consecutive list entries with one
+// atomic, so a full pass over ~10^6 tiles costs ~10^4 same-address atomics
+// instead of 10^6, while late, short lists still spread one tile per group.
 struct QueueCursor {
   int count;
+  int chunk;
   int *head;
   __device__ __forceinline__ explicit QueueCursor(const Queue &q) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -577,6 +667,21 @@ struct QueueCursor {
     }
     count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[2 * q.slot];
     head = q.ctr + 2 * q.slot + 1;
+    chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
+  }
+  // Next list index for the whole workgroup (idx/end are uniform); false
+  // once the list is drained.  Between two calls the caller executes at
+  // least one barrier, so s_idx is never overwritten while being read.
+  __device__ __forceinline__ bool next(int &idx, int &end, int *s_idx) {
+    if (idx + 1 < end) {
+      ++idx;
+      return true;
+    }
+    if (threadIdx.x == 0) *s_idx = atomicAdd(head, chunk);
+    __syncthreads();
+    idx = *s_idx;
+    end = min(idx + chunk, count);
+    return idx < count;
   }
 };
 
@@ -595,7 +700,8 @@ struct QueueCursor {
 // max_iters iterations.
 template <class S>
 __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
-                                              unsigned char *flag, int max_iters) {
+                                              unsigned char *flag, int max_iters,
+                                              long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -604,17 +710,20 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
   const Lane<S> L(tid);
   const QueueCursor Q(q);
   const long long st = P.stride;
-  for (;;) {
-    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx >= Q.count) break;
+  QueueCursor Qc = Q;
+  int idx = 0, end = 0;
+  while (Qc.next(idx, end, &s_idx)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
+    // A valid label never exceeds the distance to a deficit node, which is
+    // below the node count; a label that reaches it proves the node cut off
+    // (the role of cl_max_height in graphcut_push_relabel.cl, made exact).
+    const long long nn = (long long)I.nx * I.ny * I.nz;
+    const int hcap = nn < HINF ? (int)nn : HINF;
     long long gv = 0;
     int e0 = 0, h0 = HINF;
     int c0r[S::KMAX];
@@ -660,6 +769,7 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d)
             if (d < K && s_c[d][L.me] > 0) m = min(m, s_h[L.me + lds_off<S>(d)] + 1);
+          if (m >= hcap) m = HINF;
           if (m > h) s_h[L.me] = m;
         }
       }
@@ -693,6 +803,8 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
       }
       any = __syncthreads_or(pushed);
     } while (any && ++it < max_iters);
+    if (tid == 0 && iter_total)
+      atomicAdd((unsigned long long *)iter_total, (unsigned long long)(it + (any ? 0 : 1)));
 
     // write back what changed
     int act = 0;
@@ -733,8 +845,8 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, int max_iters,
-                 hipStream_t st) {
-  JFC_DISPATCH(shape, (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, max_iters)));
+                 long long *iter_total, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, max_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
@@ -745,18 +857,17 @@ void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
 // deficit node (graphcut_global_relabel.cl:50-124).
 template <class S>
 __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
-                                               unsigned char *dirty) {
+                                               unsigned char *dirty,
+                                               unsigned char *active) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_idx;
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   const QueueCursor Q(q);
   const long long st = P.stride;
-  for (;;) {
-    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx >= Q.count) break;
+  QueueCursor Qc = Q;
+  int idx = 0, end = 0;
+  while (Qc.next(idx, end, &s_idx)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
@@ -764,11 +875,12 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
     long long gv = 0;
-    int h0 = HINF;
+    int h0 = HINF, e0 = 0;
     unsigned mask = 0;
     if (valid) {
       gv = node_index(I, gx, gy, gz);
       h0 = P.h[gv];
+      e0 = P.e[gv];
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
         if (d < K && P.cap[d * st + gv] > 0) mask |= 1u << d;
@@ -821,13 +933,17 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
         }
       }
     }
-    __syncthreads();
+    // graphcut_count_node fused in: the tile's last visit leaves its
+    // active flag (a node with excess and a finite height)
+    const int act = __syncthreads_or(valid && e0 > 0 && s_h[L.me] < HINF);
+    if (tid == 0) active[t] = (unsigned char)act;
   }
 }
 
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
-                  const Queue &q, unsigned char *dirty, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_relax<S><<<nwg, 256, 0, st>>>(T, P, q, dirty)));
+                  const Queue &q, unsigned char *dirty, unsigned char *active,
+                  hipStream_t st) {
+  JFC_DISPATCH(shape, (k_relax<S><<<nwg, 256, 0, st>>>(T, P, q, dirty, active)));
 }
 
 // ---- canonical cut: residual reachability from excess nodes -------------------------
@@ -847,11 +963,9 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
   const Lane<S> L(tid);
   const QueueCursor Q(q);
   const long long st = P.stride;
-  for (;;) {
-    if (tid == 0) s_idx = atomicAdd(Q.head, 1);
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx >= Q.count) break;
+  QueueCursor Qc = Q;
+  int idx = 0, end = 0;
+  while (Qc.next(idx, end, &s_idx)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;

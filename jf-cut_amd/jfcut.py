@@ -35,7 +35,8 @@ class Options(ctypes.Structure):
     _fields_ = [("max_sweeps", ctypes.c_int), ("check_every", ctypes.c_int),
                 ("relabel_work", ctypes.c_double), ("max_local_iters", ctypes.c_int),
                 ("tile_shape", ctypes.c_int), ("profile", ctypes.c_int),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("initial_relabel", ctypes.c_int),
+                ("stall_ratio", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -47,7 +48,7 @@ class Stats(ctypes.Structure):
                 ("relabel_tiles", ctypes.c_int64), ("cut_tiles", ctypes.c_int64),
                 ("push_bytes", ctypes.c_int64), ("ms_total", ctypes.c_double),
                 ("ms_push", ctypes.c_double), ("ms_relabel", ctypes.c_double),
-                ("ms_cut", ctypes.c_double)]
+                ("ms_cut", ctypes.c_double), ("local_iters", ctypes.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -80,7 +81,7 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_get_flows", "jfc_get_labels", "jfc_get_energy", "jfc_backward",
            "jfc_model_image", "jfc_synth_image", "jfc_synth_video", "jfc_synth_volume",
            "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
-           "jfc_debug_tile_order"]
+           "jfc_debug_tile_order", "jfc_debug_sweep_counts"]
 
 
 def lib():
@@ -118,6 +119,7 @@ def lib():
         "jfc_get_neighbor_caps": (I, [P, I, I, P, I]),
         "jfc_debug_tiles": (I, [P, I, P, P, P, P]),
         "jfc_debug_tile_order": (I, [P, P, P, P, P]),
+        "jfc_debug_sweep_counts": (I, [P, P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -303,6 +305,13 @@ class GridGraph:
         _check(lib().jfc_debug_tile_order(self._h, vp(seed), vp(dist), vp(order),
                                           ctypes.byref(mx)))
         return dist, order, mx.value
+
+    def debug_sweep_counts(self, n=1 << 16):
+        out = np.zeros(n, np.int32)
+        m = lib().jfc_debug_sweep_counts(self._h, out.ctypes.data_as(ctypes.c_void_p), n)
+        if m < 0:
+            _check(m)
+        return out[:min(m, n)]
 
     def energy(self, inst=0):
         e = ctypes.c_int64()
