@@ -55,6 +55,77 @@ def build_graph(wl, seed=0, W=None, H=None):
     return g
 
 
+class DeviceBuffer:
+    """Raw HIP allocation (no torch needed); exposes data_ptr() like a tensor."""
+
+    def __init__(self, nbytes):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.p = ctypes.c_void_p()
+        rc = self.hip.hipMalloc(ctypes.byref(self.p), ctypes.c_size_t(max(1, nbytes)))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed: {rc}")
+
+    def data_ptr(self):
+        return self.p.value
+
+    def free(self):
+        if self.p:
+            self.hip.hipFree(self.p)
+            self.p = ctypes.c_void_p()
+
+
+def batched_forward_backward(n=256, reps=3):
+    """Config 5: the 256 uneven cuts (4C/8C, lambda 20-80) solved together and
+    differentiated in one backward launch, against the same cuts solved and
+    differentiated one graph at a time.  Flows must agree exactly."""
+    specs = jfcut.batch_specs(n, 0)
+    shapes = [(s["W"], s["H"], 1, s["conn"]) for s in specs]
+    G = jfcut.GridGraph(shapes)
+    singles = []
+    for i, s in enumerate(specs):
+        img, sd = jfcut.synth_image(s["W"], s["H"], nblobs=8, noise=25.0, seed=s["seed"])
+        G.model_image(img, sd, inst=i, k_fg=5, k_bg=5, lambda_xy=s["lam"])
+        g = jfcut.GridGraph([shapes[i]])
+        g.model_image(img, sd, k_fg=5, k_bg=5, lambda_xy=s["lam"])
+        singles.append(g)
+    ntot, kmax = G.size()
+    gb = [DeviceBuffer(ntot * 4), DeviceBuffer(ntot * 4), DeviceBuffer(ntot * kmax * 4)]
+    maxn = max(s["W"] * s["H"] for s in specs)
+    gs = [DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 8 * 4)]
+    G.solve()
+    for g in singles:
+        g.solve()
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = G.solve()
+        G.backward_all(out=gb)
+    hip_sync()
+    t_batch = (time.perf_counter() - t0) / reps
+    flows_b = G.flows()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for g in singles:
+            g.solve()
+            g.backward(out=gs)
+    hip_sync()
+    t_seq = (time.perf_counter() - t0) / reps
+    flows_s = np.array([g.flow() for g in singles])
+    assert np.array_equal(flows_b, flows_s), "batched and one-at-a-time flows differ"
+    for g in singles:
+        g.close()
+    G.close()
+    for b in gb + gs:
+        b.free()
+    return {"instances": n, "nodes": int(ntot),
+            "batched_images_per_s": round(n / t_batch, 1),
+            "sequential_images_per_s": round(n / t_seq, 1),
+            "speedup": round(t_seq / t_batch, 2),
+            "batched_mvoxels_per_s": round(ntot / t_batch / 1e6, 1),
+            "batched_ms": round(t_batch * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
+            "batched_sweeps": st.sweeps, "flows_equal": True}
+
+
 def cpu_baseline(wl, sample_w, sample_h):
     """The reference's CPU max-flow (BK303 built from the reference sources)
     on a bounded sample of the workload, plus the GPU solve of the same sample
@@ -97,6 +168,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-sample", default="7500x5000")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -134,6 +206,11 @@ def main():
     achieved = push_bytes / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0
     g.close()
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_forward_backward()
+        log(f"batched fwd+bwd: {batched}")
+
     cpu = None
     if not args.no_cpu_baseline:
         sw, sh = [int(v) for v in args.cpu_sample.split("x")]
@@ -163,6 +240,7 @@ def main():
                      "bytes_per_launch": int(push_bytes / max(1, push_launches)),
                      "avg_launch_ms": round(push_ms / max(1, push_launches), 4)},
         "cpu_baseline": cpu,
+        "batched_fwd_bwd": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -194,6 +194,17 @@ int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *cap,
 int jfc_backward(jfc_graph *g, int inst, float *d_cap_source, float *d_cap_sink,
                  float *d_neighbor, int on_device);
 
+/* The same for every instance of a graph in one launch: outputs use the
+ * graph's concatenated node numbering (instance i starts after the nodes of
+ * instances 0..i-1), d_neighbor holds Kmax planes of that many nodes (Kmax =
+ * largest connectivity of the graph; planes beyond an instance's own
+ * connectivity are left untouched). */
+int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_sink,
+                     float *d_neighbor, int on_device);
+
+/* Total nodes of the graph (all instances) and the largest connectivity. */
+int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
+
 /* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
 
 typedef struct {

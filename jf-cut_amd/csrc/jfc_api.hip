@@ -821,6 +821,43 @@ extern "C" int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int3
   return JFC_OK;
 }
 
+extern "C" int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  if (nodes) *nodes = g->Ntot;
+  if (kmax) *kmax = g->Kg;
+  return JFC_OK;
+}
+
+extern "C" int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_sink,
+                                float *d_neighbor, int on_device) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  const long long n = g->Ntot;
+  if (on_device) {
+    launch_backward_all(g->shape, g->tiles(), g->planes(), d_cap_source, d_cap_sink,
+                        d_neighbor, 0);
+    JFC_HIP(hipDeviceSynchronize());
+    return JFC_OK;
+  }
+  int rc;
+  float *a = nullptr, *b = nullptr, *c = nullptr;
+  if ((d_cap_source && (rc = dalloc(&a, n, "gradient"))) ||
+      (d_cap_sink && (rc = dalloc(&b, n, "gradient"))) ||
+      (d_neighbor && (rc = dalloc(&c, n * g->Kg, "gradient")))) {
+    dfree(a); dfree(b); dfree(c);
+    return rc;
+  }
+  if (c) JFC_HIP(hipMemcpy(c, d_neighbor, n * g->Kg * 4, hipMemcpyHostToDevice));
+  launch_backward_all(g->shape, g->tiles(), g->planes(), a, b, c, 0);
+  hipError_t err = hipDeviceSynchronize();
+  if (err == hipSuccess && a) err = hipMemcpy(d_cap_source, a, n * 4, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && b) err = hipMemcpy(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && c) err = hipMemcpy(d_neighbor, c, n * g->Kg * 4, hipMemcpyDeviceToHost);
+  dfree(a); dfree(b); dfree(c);
+  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
+  return JFC_OK;
+}
+
 extern "C" int jfc_debug_sweep_counts(jfc_graph *g, int32_t *counts, int32_t n) {
   if (!g || !counts) return fail(JFC_ERR_INVALID, "null argument");
   const int m = std::min(n, g->last_sweeps);

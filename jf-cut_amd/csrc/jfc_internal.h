@@ -112,5 +112,7 @@ void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
 void launch_backward(int shape, const Tiles &T, const Planes &P,
                      const InstDesc &I, float *dcs, float *dct, float *dnb,
                      hipStream_t st);
+void launch_backward_all(int shape, const Tiles &T, const Planes &P, float *dcs,
+                         float *dct, float *dnb, hipStream_t st);
 
 }  // namespace jfc

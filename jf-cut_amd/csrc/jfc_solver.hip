@@ -332,7 +332,8 @@ void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
 template <class S>
 __global__ __launch_bounds__(256) void k_backward(Tiles T, Planes P, int t_begin,
                                                   int t_count, float *dcs,
-                                                  float *dct, float *dnb) {
+                                                  float *dct, float *dnb,
+                                                  int all_instances) {
   const Lane<S> L(threadIdx.x);
   for (int t = t_begin + blockIdx.x; t < t_begin + t_count; t += gridDim.x) {
     const TileRef r = tile_ref<S>(T, t);
@@ -340,8 +341,10 @@ __global__ __launch_bounds__(256) void k_backward(Tiles T, Planes P, int t_begin
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     if (!(gx < I.nx && gy < I.ny && gz < I.nz)) continue;
     const long long v = node_index(I, gx, gy, gz);
-    const long long lv = v - I.node_off;
-    const long long n = (long long)I.nx * I.ny * I.nz;
+    // one instance: its own numbering and plane stride; all instances: the
+    // graph's concatenated numbering with plane stride = total nodes
+    const long long lv = all_instances ? v : v - I.node_off;
+    const long long n = all_instances ? P.stride : (long long)I.nx * I.ny * I.nz;
     const bool s = P.lab[v] != 0;
     if (dcs) dcs[lv] = s ? 0.f : 1.f;
     if (dct) dct[lv] = s ? 1.f : 0.f;
@@ -365,7 +368,13 @@ void launch_backward(int shape, const Tiles &T, const Planes &P,
                      const InstDesc &I, float *dcs, float *dct, float *dnb,
                      hipStream_t st) {
   JFC_DISPATCH(shape, (k_backward<S><<<tile_grid(I.ntiles), 256, 0, st>>>(
-                          T, P, I.tile_off, I.ntiles, dcs, dct, dnb)));
+                          T, P, I.tile_off, I.ntiles, dcs, dct, dnb, 0)));
+}
+
+void launch_backward_all(int shape, const Tiles &T, const Planes &P, float *dcs,
+                         float *dct, float *dnb, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_backward<S><<<tile_grid(T.ntiles), 256, 0, st>>>(
+                          T, P, 0, T.ntiles, dcs, dct, dnb, 1)));
 }
 
 // ---- initialisation of relabel / cut searches --------------------------------

@@ -81,7 +81,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_get_flows", "jfc_get_labels", "jfc_get_energy", "jfc_backward",
            "jfc_model_image", "jfc_synth_image", "jfc_synth_video", "jfc_synth_volume",
            "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
-           "jfc_debug_tile_order", "jfc_debug_sweep_counts"]
+           "jfc_debug_tile_order", "jfc_debug_sweep_counts", "jfc_backward_all",
+           "jfc_graph_size"]
 
 
 def lib():
@@ -120,6 +121,8 @@ def lib():
         "jfc_debug_tiles": (I, [P, I, P, P, P, P]),
         "jfc_debug_tile_order": (I, [P, P, P, P, P]),
         "jfc_debug_sweep_counts": (I, [P, P, I]),
+        "jfc_backward_all": (I, [P, P, P, P, I]),
+        "jfc_graph_size": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -271,6 +274,29 @@ class GridGraph:
         _check(lib().jfc_get_labels(self._h, inst, lab.ctypes.data_as(ctypes.c_void_p), 0))
         return lab
 
+    def size(self):
+        """(total nodes over all instances, largest connectivity)."""
+        n, k = ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().jfc_graph_size(self._h, ctypes.byref(n), ctypes.byref(k)))
+        return n.value, k.value
+
+    def backward_all(self, out=None):
+        """Gradients of every instance in one launch, concatenated node order:
+        (d_source[Ntot], d_sink[Ntot], d_neighbor[Kmax, Ntot]); into GPU tensors
+        when `out` is given."""
+        if out is not None:
+            ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+            a, b, c = out
+            _check(lib().jfc_backward_all(self._h, ptr(a), ptr(b), ptr(c), 1))
+            return out
+        n, K = self.size()
+        a = np.zeros(n, np.float32)
+        b = np.zeros(n, np.float32)
+        c = np.zeros((K, n), np.float32)
+        vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _check(lib().jfc_backward_all(self._h, vp(a), vp(b), vp(c), 0))
+        return a, b, c
+
     def terminal_caps(self, inst=0):
         n = self.nodes(inst)
         cs, ct = np.zeros(n, np.int32), np.zeros(n, np.int32)
@@ -336,6 +362,69 @@ class GridGraph:
 
 
 # ---- synthetic workloads --------------------------------------------------------
+
+# The five benchmark configurations: generator and data-term parameters.
+CONFIGS = {
+    "c1": dict(kind="image", W=512, H=512, conn=4, nblobs=8, noise=25.0, k=5, lam=50.0,
+               lam_z=50.0, z_sep=0, rng=255),
+    "c2": dict(kind="image", W=15000, H=10000, conn=8, nblobs=40, noise=25.0, k=5, lam=50.0,
+               lam_z=50.0, z_sep=0, rng=255),
+    "c3": dict(kind="video", W=2560, H=1440, T=50, conn=6, nblobs=8, noise=20.0, k=5,
+               lam=50.0, lam_z=30.0, z_sep=1, rng=255),
+    "c4": dict(kind="volume", W=512, H=512, D=512, conn=6, nell=12, k=4, lam=40.0,
+               lam_z=40.0, z_sep=0, rng=1024),
+}
+
+
+def _splitmix(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def batch_specs(n=256, seed=0):
+    """Instances of the batched configuration: side length log-uniform in
+    [64, 1024], aspect ratio uniform in [0.5, 2] (W = s*sqrt(a), H = s/sqrt(a)),
+    4- or 8-connectivity with p = 0.5, lambda uniform in [20, 80].
+    Returns dicts (W, H, conn, lam, seed); deterministic in `seed`."""
+    import math
+    st = seed
+    out = []
+    for i in range(n):
+        u = []
+        for _ in range(4):
+            st, z = _splitmix(st)
+            u.append((z >> 11) / float(1 << 53))
+        s = math.exp(math.log(64.0) + u[0] * (math.log(1024.0) - math.log(64.0)))
+        a = 0.5 + 1.5 * u[1]
+        W = max(32, int(round(s * math.sqrt(a))))
+        H = max(32, int(round(s / math.sqrt(a))))
+        conn = 4 if u[2] < 0.5 else 8
+        lam = 20.0 + 60.0 * u[3]
+        out.append(dict(W=W, H=H, conn=conn, lam=lam, seed=1000 + i))
+    return out
+
+
+def build_config(name, seed=0, inst=None, graph=None):
+    """Synthesise a configuration's input and build its graph on the GPU with
+    jfc_model_image.  Returns (graph, image, seeds)."""
+    c = CONFIGS[name]
+    if c["kind"] == "image":
+        img, sd = synth_image(c["W"], c["H"], nblobs=c["nblobs"], noise=c["noise"], seed=seed)
+        shape = (c["W"], c["H"], 1, c["conn"])
+    elif c["kind"] == "video":
+        img, sd = synth_video(c["W"], c["H"], c["T"], nblobs=c["nblobs"], noise=c["noise"],
+                              seed=seed)
+        shape = (c["W"], c["H"], c["T"], c["conn"])
+    else:
+        img, sd = synth_volume(c["W"], c["H"], c["D"], nell=c["nell"], seed=seed)
+        shape = (c["W"], c["H"], c["D"], c["conn"])
+    g = graph if graph is not None else GridGraph([shape])
+    g.model_image(img, sd, inst=inst or 0, k_fg=c["k"], k_bg=c["k"], rng=c["rng"],
+                  lambda_xy=c["lam"], lambda_z=c["lam_z"], z_separate=c["z_sep"])
+    return g, img, sd
 
 def synth_image(W, H, nblobs=8, noise=25.0, seed=0):
     rgb = np.zeros((H, W, 3), np.uint8)
