@@ -169,12 +169,26 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB, the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-sample", default="7500x5000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
               "slab sharding is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+        hip = ctypes.CDLL("libamdhip64.so")
+        rc = hip.hipMemcpy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
+                           ctypes.c_size_t(nb), 3)  # hipMemcpyDeviceToDevice
+        hip_sync()
+        a.free()
+        b.free()
+        log(f"calibration copy of {args.calibrate_copy_mib} MiB: rc {rc}")
 
     wl = WORKLOADS[args.workload]
     N = wl["W"] * wl["H"]
@@ -205,6 +219,15 @@ def main():
     push_launches = sum(s.push_launches for s in stats)
     achieved = push_bytes / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0
     g.close()
+    # HBM bytes per k_push dispatch from the calibrated rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (profiles/k_push_c2_15000x10000_8c_traffic.md)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_push_c2_traffic.json")
+    if args.workload == "c2" and os.path.exists(tpath):
+        with open(tpath) as f:
+            kt = [v for k, v in json.load(f)["kernels"].items() if "k_push" in k]
+        if kt:
+            traffic = kt[0]["read_bytes_per_dispatch"] + kt[0]["write_bytes_per_dispatch"]
 
     batched = None
     if not args.no_batched:
@@ -235,7 +258,7 @@ def main():
                    "sweeps": stats[-1].sweeps, "global_relabels": stats[-1].global_relabels,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_push (block-wise push-relabel)",
                      "bytes_per_launch": int(push_bytes / max(1, push_launches)),
                      "avg_launch_ms": round(push_ms / max(1, push_launches), 4)},

@@ -74,8 +74,9 @@ typedef struct {
   int initial_relabel;    /* 1 = exact global relabel before the first sweep; 0 =
                              start from zero heights (the reference's default
                              "no global relabel", QGCW:2394-2401) */
-  double stall_ratio;     /* run a global relabel when a sweep visits at least this
-                             fraction of the previous sweep's tiles (0 = off) */
+  double stall_ratio;     /* run a global relabel when the active-tile count of a
+                             sweep stays within [r, 1/r] of the previous sweep's
+                             (the active set has stopped changing; 0 = off) */
 } jfc_options;
 
 #define JFC_TILE_DEFAULT 0

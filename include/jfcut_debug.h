@@ -31,6 +31,13 @@ int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims 
 int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int32_t *tile_dist,
                          int32_t *order, int32_t *max_dist);
 
+/* Per-tile flags computed by the device from the current input capacities
+ * (folded e = cs - ct): which = 0 -> tiles holding a deficit node (relabel
+ * seeds, graphcut_init_global_relabel), 1 -> tiles holding an excess node
+ * (cut seeds, graphcut_init_cut), 2 -> active tiles at zero heights
+ * (graphcut_count_node after the clear of h).  ntiles bytes, host output. */
+int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags);
+
 /* Active tiles visited by each push-relabel sweep of the last solve (all
  * colour classes of the sweep together); returns the number of sweeps, writes
  * at most n counts. */

@@ -244,7 +244,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->profile = 0;
   o->stream = nullptr;
   o->initial_relabel = 0;
-  o->stall_ratio = 0.97;
+  o->stall_ratio = 0.99;
   o->check_every = 2;
   o->max_local_iters = 8;
   o->relabel_work = 2.0;
@@ -510,6 +510,13 @@ struct Solver {
     JFC_HIP(hipStreamSynchronize(st));
     return JFC_OK;
   }
+  int read_ll(const long long *dptr, long long *out) {
+    JFC_HIP(hipMemcpyAsync(out, dptr, sizeof(long long), hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipStreamSynchronize(st));
+    return JFC_OK;
+  }
+  // relabel / cut passes issued between two host checks
+  static constexpr int kPassGroup = 4;
 
   // Tile passes of a monotone search (relabel or cut) until no tile is dirty.
   // The first pass visits every tile in jump-flooding distance order; later
@@ -521,29 +528,42 @@ struct Solver {
     launch_tile_order(g->shape, T, g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist, g->d_hist,
                       g->nbins, g->d_order, g->max_tile_dim, nullptr, st);
     launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
-    Queue q{g->d_order, g->d_ctr, g->slot, g->ntiles};
     const int nwg = std::min(g->nwg, g->ntiles);
-    if (cut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
-    else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
+    long long *total = g->d_total + (cut ? 2 : 1);
+    long long before = 0;
+    if (int rc = read_ll(total, &before)) return rc;
+    if (cut) {
+      // tiles without excess gain labels only through their halo, so the
+      // first pass visits the seed tiles (in distance order)
+      Queue q = queue_from_list();
+      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, q, nullptr, total, st);
+      launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
+    } else {
+      Queue q{g->d_order, g->d_ctr, g->slot, g->ntiles};
+      launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
+    }
     advance();
     *passes += 1;
-    *tiles += g->ntiles;
-    for (int pass = 1;; ++pass) {
-      Queue qc = queue_from_list();
-      launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, qc, nullptr,
-                     g->d_total + (cut ? 2 : 1), st);
+    // later passes over the dirty tiles, alternating direction, issued in
+    // groups between host checks; an empty compaction ends the search
+    for (int pass = 1;;) {
+      for (int k = 0; k < kPassGroup; ++k, ++pass) {
+        Queue qc = queue_from_list();
+        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, qc, nullptr, total, st);
+        if (cut) launch_reach(g->shape, nwg, T, P, qc, g->d_dirty, g->d_flagword, st);
+        else launch_relax(g->shape, nwg, T, P, qc, g->d_dirty, g->d_flag, st);
+        advance();
+        *passes += 1;
+      }
       int cnt = 0;
-      int rc = read_int(g->d_ctr + 2 * g->slot, &cnt);
-      if (rc) return rc;
+      // the last compaction's slot is the one before the current
+      if (int rc = read_int(g->d_ctr + 2 * (g->slot ^ 1), &cnt)) return rc;
       if (cnt == 0) break;
-      const int w = std::min(g->nwg, cnt);
-      if (cut) launch_reach(g->shape, w, T, P, qc, g->d_dirty, g->d_flagword, st);
-      else launch_relax(g->shape, w, T, P, qc, g->d_dirty, g->d_flag, st);
-      advance();
-      *passes += 1;
-      *tiles += cnt;
       if (pass > 1000000) return fail(JFC_ERR_STATE, "search did not settle");
     }
+    long long after = 0;
+    if (int rc = read_ll(total, &after)) return rc;
+    *tiles += after - before + (cut ? 0 : g->ntiles);
     return JFC_OK;
   }
 
@@ -591,7 +611,7 @@ struct Solver {
     g->slot = 0;
     JFC_HIP(hipMemcpyAsync(g->cap, g->c0, (size_t)g->Ntot * g->Kg * sizeof(int),
                            hipMemcpyDeviceToDevice, st));
-    launch_fold(g->shape, T, P, g->d_red, st);
+    launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
     std::vector<long long> red((size_t)nI * RED_SLOTS);
     JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
@@ -604,11 +624,7 @@ struct Solver {
     int rc;
     if (o.initial_relabel) {
       if ((rc = global_relabel())) return rc;
-    } else {
-      // zero heights are a valid labelling (QGCW:2400 clears h)
-      launch_fill_i32(g->h, g->Ntot, 0, st);
-      launch_count_active(g->shape, T, P, g->d_flag, st);
-    }
+    }  // else k_fold already set zero heights (a valid labelling) and the flags
     long long total_at_gr = 0;
     int since_check = 0;
     int cert_rounds = 0;
@@ -648,8 +664,11 @@ struct Solver {
           // relabel when the active set stops shrinking (nodes cut off from
           // every deficit keep sloshing until their heights are recomputed),
           // or when the work since the last relabel exceeds the budget
+          // stalled: the active set neither shrinks nor grows by more than
+          // (1 - stall_ratio); a growing set is flow still spreading
+          const double prev = recent.size() >= 2 ? recent[recent.size() - 2] : 0.0;
           const bool stalled = o.stall_ratio > 0 && recent.size() >= 2 &&
-                               cnt >= o.stall_ratio * recent[recent.size() - 2];
+                               cnt >= o.stall_ratio * prev && cnt * o.stall_ratio <= prev;
           const bool budget = o.relabel_work > 0 &&
                               (double)(total - total_at_gr) > o.relabel_work * g->ntiles;
           if (stalled || budget) {
@@ -818,6 +837,20 @@ extern "C" int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int3
   if (tile_dist) JFC_HIP(hipMemcpy(tile_dist, g->d_tdist, g->ntiles * 4, hipMemcpyDeviceToHost));
   if (order) JFC_HIP(hipMemcpy(order, g->d_order, g->ntiles * 4, hipMemcpyDeviceToHost));
   if (max_dist) JFC_HIP(hipMemcpy(max_dist, g->d_flagword + 2, 4, hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+
+extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
+  if (!g || !flags || which < 0 || which > 2) return fail(JFC_ERR_INVALID, "bad argument");
+  const int nI = (int)g->inst.size();
+  launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, 0);
+  launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr, 0);
+  unsigned char *src = g->d_flag;
+  if (which == 0) { launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, 0); src = g->d_seed; }
+  if (which == 1) { launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, 0); src = g->d_seed; }
+  JFC_HIP(hipDeviceSynchronize());
+  JFC_HIP(hipMemcpy(flags, src, g->ntiles, hipMemcpyDeviceToHost));
+  g->solved = false;
   return JFC_OK;
 }
 

@@ -70,7 +70,7 @@ void launch_fill_i32(int *p, long long n, int v, hipStream_t st);
 void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t st);
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 hipStream_t st);
+                 unsigned char *active, hipStream_t st);
 void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,

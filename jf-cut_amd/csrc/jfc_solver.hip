@@ -158,6 +158,22 @@ static int tile_grid(int ntiles) { return ntiles < 4096 ? (ntiles > 0 ? ntiles :
 // latency with one 256-lane group per tile.
 constexpr int TU = 4;
 
+// Bitwise OR of TU per-tile bits over the 256-lane workgroup (4 waves):
+// one ballot per bit, one LDS word per wave.  The buffer half alternates with
+// `parity`, so the words read in one group are never overwritten before
+// every lane has passed the next group's barrier.  (__syncthreads_or only
+// tells whether any lane is non-zero.)
+__device__ __forceinline__ int block_or_bits(int bits, int *s_w, int parity) {
+  int m = 0;
+#pragma unroll
+  for (int u = 0; u < TU; ++u)
+    if (__ballot((bits >> u) & 1)) m |= 1 << u;
+  int *buf = s_w + 4 * parity;
+  if (lane_id() == 0) buf[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return buf[0] | buf[1] | buf[2] | buf[3];
+}
+
 // Node of lane L in tile t; inst = -1 past the end of the chunk.
 struct TileNode {
   int inst;
@@ -192,11 +208,16 @@ __device__ __forceinline__ void fold_flush(long long *red, int inst, long long a
 
 // Fold terminal arcs: e = cs - ct; const += min(cs, ct);
 // deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
+// With `active` set it also starts the labelling at zero heights (the
+// reference clears h, QGCWidget.cpp:2400) and flags the tiles holding excess.
 template <class S>
-__global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red) {
+__global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
+                                              unsigned char *active) {
   __shared__ long long buf[4];
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
+  __shared__ int s_w[8];
+  int par = 0;
   int cur = -1;
   long long a = 0, b = 0, c = 0;
   for (int t = C.t0; t < C.t1; t += TU) {
@@ -209,6 +230,7 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red)
       s[u] = nd[u].valid ? P.cs[nd[u].v] : 0;
       k[u] = nd[u].valid ? P.ct[nd[u].v] : 0;
     }
+    int bits = 0;
 #pragma unroll
     for (int u = 0; u < TU; ++u) {
       if (nd[u].inst < 0) break;
@@ -219,18 +241,25 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red)
       }
       if (nd[u].valid) {
         P.e[nd[u].v] = s[u] - k[u];
+        if (active) P.h[nd[u].v] = 0;
+        bits |= (s[u] > k[u]) << u;
         a += min(s[u], k[u]);
         b += max(k[u] - s[u], 0);
         c += max(s[u] - k[u], 0);
       }
+    }
+    if (active) {
+      bits = block_or_bits(bits, s_w, par ^= 1);
+      if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
+        active[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
     }
   }
   if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
 }
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 hipStream_t st) {
-  JFC_DISPATCH(shape, (k_fold<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+                 unsigned char *active, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_fold<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red, active)));
 }
 
 // Remaining deficit per instance: sum of max(-e, 0).
@@ -385,6 +414,8 @@ template <class S>
 __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
+  __shared__ int s_w[8];
+  int par = 0;
   for (int t = C.t0; t < C.t1; t += TU) {
     TileNode nd[TU];
     int e[TU];
@@ -399,7 +430,7 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned cha
         P.h[nd[u].v] = e[u] < 0 ? 0 : HINF;
         bits |= (e[u] < 0) << u;
       }
-    bits = __syncthreads_or(bits);
+    bits = block_or_bits(bits, s_w, par ^= 1);
     if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
       seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
@@ -416,6 +447,8 @@ template <class S>
 __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned char *seed) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
+  __shared__ int s_w[8];
+  int par = 0;
   for (int t = C.t0; t < C.t1; t += TU) {
     TileNode nd[TU];
     int e[TU];
@@ -430,7 +463,7 @@ __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned ch
         P.lab[nd[u].v] = (unsigned char)(e[u] > 0);
         bits |= (e[u] > 0) << u;
       }
-    bits = __syncthreads_or(bits);
+    bits = block_or_bits(bits, s_w, par ^= 1);
     if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
       seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
@@ -448,6 +481,8 @@ __global__ __launch_bounds__(256) void k_count_active(Tiles T, Planes P,
                                                       unsigned char *flag) {
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
+  __shared__ int s_w[8];
+  int par = 0;
   for (int t = C.t0; t < C.t1; t += TU) {
     TileNode nd[TU];
     int e[TU], h[TU];
@@ -461,7 +496,7 @@ __global__ __launch_bounds__(256) void k_count_active(Tiles T, Planes P,
     int bits = 0;
 #pragma unroll
     for (int u = 0; u < TU; ++u) bits |= (e[u] > 0 && h[u] < HINF) << u;
-    bits = __syncthreads_or(bits);
+    bits = block_or_bits(bits, s_w, par ^= 1);
     if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
       flag[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
   }
@@ -903,8 +938,13 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
       int hv = HINF;
       if (halo_needed<S>(nout, K)) {
         const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
+        // device-coherent read: a neighbour tile settled earlier in this
+        // same pass (possibly on another XCD) is seen at once, so one pass
+        // carries the distance front across many tiles in list order.  Any
+        // value read is an upper bound of the distance, so stale is safe.
         if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
-          hv = P.h[node_index(I, X, Y, Z)];
+          hv = __hip_atomic_load(&P.h[node_index(I, X, Y, Z)], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
       }
       s_h[p] = hv;
     }
@@ -928,7 +968,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     if (valid) {
       const int h1 = s_h[L.me];
       if (h1 != h0) {
-        P.h[gv] = h1;
+        __hip_atomic_store(&P.h[gv], h1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d) {
           if (d >= K) break;

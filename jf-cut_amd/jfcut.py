@@ -82,7 +82,7 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_model_image", "jfc_synth_image", "jfc_synth_video", "jfc_synth_volume",
            "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
            "jfc_debug_tile_order", "jfc_debug_sweep_counts", "jfc_backward_all",
-           "jfc_graph_size"]
+           "jfc_graph_size", "jfc_debug_tile_flags"]
 
 
 def lib():
@@ -122,6 +122,7 @@ def lib():
         "jfc_debug_tile_order": (I, [P, P, P, P, P]),
         "jfc_debug_sweep_counts": (I, [P, P, I]),
         "jfc_backward_all": (I, [P, P, P, P, I]),
+        "jfc_debug_tile_flags": (I, [P, I, P]),
         "jfc_graph_size": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
@@ -331,6 +332,12 @@ class GridGraph:
         _check(lib().jfc_debug_tile_order(self._h, vp(seed), vp(dist), vp(order),
                                           ctypes.byref(mx)))
         return dist, order, mx.value
+
+    def debug_tile_flags(self, which):
+        nt = self.debug_tiles(0)[0]
+        out = np.zeros(nt, np.uint8)
+        _check(lib().jfc_debug_tile_flags(self._h, which, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
 
     def debug_sweep_counts(self, n=1 << 16):
         out = np.zeros(n, np.int32)

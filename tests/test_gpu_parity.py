@@ -201,6 +201,33 @@ def test_input_validation_gpu(gpu):
     g.close()
 
 
+@pytest.mark.parametrize("shape", [(203, 77, 1, 4), (129, 95, 1, 8), (37, 29, 23, 6)])
+def test_tile_flags_gpu(gpu, shape):
+    """Per-tile seed / active flags from the streaming init kernels (several
+    tiles per lane, workgroup-wide bitwise OR) against numpy."""
+    rng = np.random.default_rng(31)
+    nx, ny, nz, conn = shape
+    N = nx * ny * nz
+    # sparse terminals so that most tiles have neither excess nor deficit
+    cs = np.zeros(N, np.int32)
+    ct = np.zeros(N, np.int32)
+    cs[rng.integers(0, N, N // 300)] = 7
+    ct[rng.integers(0, N, N // 300)] = 5
+    g = gpu.GridGraph([shape])
+    g.set_terminal_caps(cs, ct)
+    nt, (tw, th, td), (gx, gy, gz), _ = g.debug_tiles(0)
+    e = (cs.astype(np.int64) - ct).reshape(nz, ny, nx)
+    pad = np.zeros((gz * td, gy * th, gx * tw), np.int64)
+    pad[:nz, :ny, :nx] = e
+    tiles = pad.reshape(gz, td, gy, th, gx, tw).transpose(0, 2, 4, 1, 3, 5).reshape(nt, -1)
+    exp_def = (tiles < 0).any(1).astype(np.uint8)
+    exp_exc = (tiles > 0).any(1).astype(np.uint8)
+    assert np.array_equal(g.debug_tile_flags(0), exp_def)
+    assert np.array_equal(g.debug_tile_flags(1), exp_exc)
+    assert np.array_equal(g.debug_tile_flags(2), exp_exc)
+    g.close()
+
+
 def test_tile_order_gpu(gpu):
     """Jump flooding over the tile grid = brute-force taxicab distance to the
     nearest seed tile; the order is sorted by it; max distance = reduce."""
