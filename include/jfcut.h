@@ -237,6 +237,46 @@ int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
                     int bytes_per_channel, const uint8_t *seeds, int on_device,
                     const jfc_model_params *params, jfc_model_stats *stats);
 
+/* ---- graph files (host only; no device work) ----------------------------- */
+
+/* MFI max-flow instance, the format of the reference's CPU benchmarks
+ * (benchmark/GridCut/mfi.h:4-77, mfi.cpp:13-193): "MFI", a packed header
+ * (dimension, connectivity, width, height, depth, int64 maxflow,
+ * symmetric_arcs, 4096 B metadata, active_count, terminal / neighbour
+ * capacity types), source and sink planes, one plane per arc in MFI arc
+ * order, and an active mask.  Capacities are read into / written from the
+ * jfcut direction order (MFI LE GE EL EG LL GL LG GG -> 1 0 3 2 5 7 6 4 in 2D;
+ * LEE GEE ELE EGE EEL EEG -> 1 0 3 2 5 4 in 3D).  Supported: 2D-4C, 2D-8C,
+ * 3D-6C. */
+int jfc_mfi_info(const char *path, int32_t *dimension, int32_t *connectivity,
+                 int32_t *nx, int32_t *ny, int32_t *nz, int64_t *maxflow);
+int jfc_mfi_read(const char *path, int32_t *cap_source, int32_t *cap_sink,
+                 int32_t *caps);
+/* Writes the smallest unsigned type that holds each capacity group; the
+ * active mask is all ones. */
+int jfc_mfi_write(const char *path, int nx, int ny, int nz, int connectivity,
+                  const int32_t *cap_source, const int32_t *cap_sink,
+                  const int32_t *caps, int64_t maxflow);
+
+/* The reference's raw graph files <prefix>.ef / .c1 / .c2
+ * (slotLoadGraphClicked / slotSaveGraphClicked, QGCW:2631-2736): int32
+ * folded terminal capacities; 2D ushort2 c1 = (+x, +y), c2 = (-y, -x);
+ * 3D uchar4 c1 = (+x, +y, +z, -), c2 = (-z, -y, -x, -).  caps holds the
+ * 4 (2D) or 6 (3D) direction planes.  Writing rejects capacities that do not
+ * fit the file's type. */
+int jfc_raw_read(const char *prefix, int nx, int ny, int nz, int32_t *ef,
+                 int32_t *caps);
+int jfc_raw_write(const char *prefix, int nx, int ny, int nz, const int32_t *ef,
+                  const int32_t *caps);
+
+/* .cut files: 4 bytes per node (foreground, background, object, tag), the
+ * cl_cut of QGCSetting.h:45-51 (QGCW:2971-2996).  Bytes are read as
+ * 0 / non-zero; written as 0 / 127.  Any pointer may be NULL. */
+int jfc_cut_read(const char *path, int64_t n, uint8_t *fg, uint8_t *bg,
+                 uint8_t *object);
+int jfc_cut_write(const char *path, int64_t n, const uint8_t *fg,
+                  const uint8_t *bg, const uint8_t *object);
+
 /* ---- synthetic workloads (benchmark inputs; no reference counterpart) ---- */
 
 /* 2D RGB image: fg ellipse at the centre (semi-axes 150x100 scaled by

@@ -82,7 +82,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_model_image", "jfc_synth_image", "jfc_synth_video", "jfc_synth_volume",
            "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
            "jfc_debug_tile_order", "jfc_debug_sweep_counts", "jfc_backward_all",
-           "jfc_graph_size", "jfc_debug_tile_flags"]
+           "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
+           "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write"]
 
 
 def lib():
@@ -124,6 +125,14 @@ def lib():
         "jfc_backward_all": (I, [P, P, P, P, I]),
         "jfc_debug_tile_flags": (I, [P, I, P]),
         "jfc_graph_size": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int32)]),
+        "jfc_mfi_info": (I, [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int32)] * 5
+                         + [ctypes.POINTER(I64)]),
+        "jfc_mfi_read": (I, [ctypes.c_char_p, P, P, P]),
+        "jfc_mfi_write": (I, [ctypes.c_char_p, I, I, I, I, P, P, P, I64]),
+        "jfc_raw_read": (I, [ctypes.c_char_p, I, I, I, P, P]),
+        "jfc_raw_write": (I, [ctypes.c_char_p, I, I, I, P, P]),
+        "jfc_cut_read": (I, [ctypes.c_char_p, I64, P, P, P]),
+        "jfc_cut_write": (I, [ctypes.c_char_p, I64, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -457,6 +466,75 @@ def synth_volume(nx, ny, nz, nell=12, seed=0):
     _check(lib().jfc_synth_volume(vol.ctypes.data_as(ctypes.c_void_p),
                                   seeds.ctypes.data_as(ctypes.c_void_p), nx, ny, nz, nell, seed))
     return vol, seeds
+
+
+# ---- graph files ------------------------------------------------------------------
+
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def read_mfi(path):
+    """MFI instance (benchmark/GridCut/mfi.cpp:13-104) as a dict: shape
+    (nx, ny, nz), connectivity, maxflow stored in the file, cs, ct of shape
+    (nz, ny, nx) and caps of shape (K, nz, ny, nx) in jfcut direction order."""
+    L = lib()
+    vals = [ctypes.c_int32() for _ in range(5)]
+    mf = ctypes.c_int64()
+    _check(L.jfc_mfi_info(os.fsencode(path), *[ctypes.byref(v) for v in vals],
+                          ctypes.byref(mf)))
+    dim, conn, nx, ny, nz = (v.value for v in vals)
+    cs = np.zeros((nz, ny, nx), np.int32)
+    ct = np.zeros_like(cs)
+    caps = np.zeros((conn, nz, ny, nx), np.int32)
+    _check(L.jfc_mfi_read(os.fsencode(path), _vp(cs), _vp(ct), _vp(caps)))
+    return {"shape": (nx, ny, nz), "connectivity": conn, "maxflow": mf.value,
+            "cs": cs, "ct": ct, "caps": caps}
+
+
+def write_mfi(path, cs, ct, caps, connectivity, maxflow=0):
+    """Write an MFI instance; cs/ct of shape (nz, ny, nx) or (ny, nx), caps
+    (K, ...) in jfcut direction order."""
+    cs = np.ascontiguousarray(cs, np.int32)
+    ct = np.ascontiguousarray(ct, np.int32)
+    caps = np.ascontiguousarray(caps, np.int32)
+    sh = cs.shape if cs.ndim == 3 else (1,) + cs.shape
+    nz, ny, nx = sh
+    _check(lib().jfc_mfi_write(os.fsencode(path), nx, ny, nz, connectivity, _vp(cs),
+                               _vp(ct), _vp(caps), int(maxflow)))
+
+
+def read_raw_graph(prefix, nx, ny, nz=1):
+    """The reference's <prefix>.ef/.c1/.c2 graph files (QGCW:2631-2683):
+    (ef, caps) with ef of shape (nz, ny, nx) and caps (K, nz, ny, nx)."""
+    K = 6 if nz > 1 else 4
+    ef = np.zeros((nz, ny, nx), np.int32)
+    caps = np.zeros((K, nz, ny, nx), np.int32)
+    _check(lib().jfc_raw_read(os.fsencode(prefix), nx, ny, nz, _vp(ef), _vp(caps)))
+    return ef, caps
+
+
+def write_raw_graph(prefix, ef, caps):
+    ef = np.ascontiguousarray(ef, np.int32)
+    caps = np.ascontiguousarray(caps, np.int32)
+    sh = ef.shape if ef.ndim == 3 else (1,) + ef.shape
+    nz, ny, nx = sh
+    _check(lib().jfc_raw_write(os.fsencode(prefix), nx, ny, nz, _vp(ef), _vp(caps)))
+
+
+def read_cut(path, n):
+    """A .cut file (QGCW:2971-2983): (fg, bg, object) as uint8 0/1 arrays."""
+    out = [np.zeros(n, np.uint8) for _ in range(3)]
+    _check(lib().jfc_cut_read(os.fsencode(path), n, *[_vp(a) for a in out]))
+    return tuple(out)
+
+
+def write_cut(path, fg=None, bg=None, obj=None):
+    arrs = [None if a is None else np.ascontiguousarray(a, np.uint8).ravel()
+            for a in (fg, bg, obj)]
+    n = next(a.size for a in arrs if a is not None)
+    _check(lib().jfc_cut_write(os.fsencode(path), n,
+                               *[None if a is None else _vp(a) for a in arrs]))
 
 
 # ---- differentiable min cut -------------------------------------------------------
