@@ -174,6 +174,15 @@ int jfc_get_flows(jfc_graph *g, int64_t *flows);
  * convergence (graphcut_compute_cut.cl:19-140). */
 int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_device);
 
+/* Tag post-process of the solved cut (graphcut_init_tag / graphcut_compute_tag,
+ * cl/GraphCut/graphcut_compute_tag.cl:19-242; the cl_cut.tag byte):
+ * tags[v] = 1 iff v is a foreground seed (fg[v] != 0) or a source-side node
+ * joined to one through axis-adjacent source-side nodes (4-adjacency in 2D
+ * whatever the graph's connectivity, 6 in 3D).  fg and tags hold the
+ * instance's nodes; both host or both device (on_device). */
+int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *tags,
+                 int on_device);
+
 /* Cut energy of the current labels on the input capacities, computed on the
  * device (QGCW:2452-2520); equals the flow once solved. */
 int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy);
@@ -273,9 +282,32 @@ int jfc_raw_write(const char *prefix, int nx, int ny, int nz, const int32_t *ef,
  * cl_cut of QGCSetting.h:45-51 (QGCW:2971-2996).  Bytes are read as
  * 0 / non-zero; written as 0 / 127.  Any pointer may be NULL. */
 int jfc_cut_read(const char *path, int64_t n, uint8_t *fg, uint8_t *bg,
-                 uint8_t *object);
+                 uint8_t *object, uint8_t *tag);
 int jfc_cut_write(const char *path, int64_t n, const uint8_t *fg,
-                  const uint8_t *bg, const uint8_t *object);
+                  const uint8_t *bg, const uint8_t *object, const uint8_t *tag);
+
+/* .node files: one record per node of a grid padded up to whole tiles of
+ * block_x x block_y x block_z (QGCW:2642-2717; records cl_node_2d / cl_node_3d
+ * of QGCSetting.h:31-43).  2D record {int32 height, int32 excess, uint16
+ * cap[4] = (+x, +y, -y, -x)}; 3D record {int32 height, int32 excess, uint8
+ * cap[8] = (+x, +y, +z, -z, -y, -x, -, -)}.  height may be NULL on read and on
+ * write (then written as 0).  Padding records are written as zeros. */
+int jfc_node_read(const char *path, int nx, int ny, int nz, int block_x,
+                  int block_y, int block_z, int32_t *ef, int32_t *height,
+                  int32_t *caps);
+int jfc_node_write(const char *path, int nx, int ny, int nz, int block_x,
+                   int block_y, int block_z, const int32_t *ef,
+                   const int32_t *height, const int32_t *caps);
+
+/* The MFI export of the reference's graph-cut widget (saveMFI, QGCW:3339-3473):
+ * folded excess ef -> (source, sink) = (max(ef, 0), max(-ef, 0)); 2D-4C with
+ * uint16 or 3D-6C with uint8 neighbour capacities, uint32 terminals;
+ * symmetric_arcs = 1; active_count = number of nodes with ef >= 0; active mask
+ * = 1 foreground seed, 2 background seed, 3 object, 4 tag, else 0 (first match
+ * in that order; any of fg/bg/object/tag may be NULL); maxflow as given. */
+int jfc_mfi_export(const char *path, int nx, int ny, int nz, const int32_t *ef,
+                   const int32_t *caps, const uint8_t *fg, const uint8_t *bg,
+                   const uint8_t *object, const uint8_t *tag, int64_t maxflow);
 
 /* ---- synthetic workloads (benchmark inputs; no reference counterpart) ---- */
 

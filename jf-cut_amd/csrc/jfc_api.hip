@@ -33,6 +33,14 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
+}  // namespace
+
+// Host-only parts of the library (graph files) report through the same
+// thread-local message.
+void jfc::set_last_error(const std::string &msg) { g_err = msg; }
+
+namespace {
+
 #define JFC_HIP(call)                                                        \
   do {                                                                       \
     hipError_t e_ = (call);                                                  \
@@ -94,6 +102,7 @@ struct jfc_graph {
   int *cs = nullptr, *ct = nullptr, *c0 = nullptr, *cap = nullptr;
   int *e = nullptr, *h = nullptr;
   unsigned char *lab = nullptr;
+  unsigned char *tag = nullptr;  // tag post-process output (allocated on first use)
   // device tables / work space
   InstDesc *d_inst = nullptr;
   int *d_tile_inst = nullptr, *d_color_tiles = nullptr;
@@ -107,6 +116,10 @@ struct jfc_graph {
   int sweep_cap = 0;
   int last_sweeps = 0;
   int *d_flagword = nullptr;  // [0] certificate, [1] validation
+  unsigned *mask0 = nullptr;  // residual-arc bits of the input planes
+  unsigned *rmask = nullptr;  // residual-arc bits of the current residual
+  int *d_fresh = nullptr;     // per tile: residual planes not yet copied from c0
+  bool mask_dirty = true;     // inputs changed since mask0 was built
   int slot = 0;
   bool solved = false;
   std::vector<long long> flows;
@@ -118,18 +131,19 @@ struct jfc_graph {
   void release_tiles() {
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
-    dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst);
+    dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_fresh);
   }
   void release() {
     release_tiles();
-    dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab);
+    dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
     dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
+    dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
   }
 
   Tiles tiles() const { return Tiles{d_inst, d_tile_inst, ntiles}; }
-  Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot}; }
+  Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot, rmask}; }
   long long inst_nodes(int i) const {
     return (long long)inst[i].nx * inst[i].ny * inst[i].nz;
   }
@@ -196,7 +210,8 @@ struct jfc_graph {
         (rc = dalloc(&d_jf1, ntiles, "jump flooding")) ||
         (rc = dalloc(&d_tdist, ntiles, "tile distance")) ||
         (rc = dalloc(&d_hist, nbins, "tile histogram")) ||
-        (rc = dalloc(&d_order, ntiles, "tile order")))
+        (rc = dalloc(&d_order, ntiles, "tile order")) ||
+        (rc = dalloc(&d_fresh, ntiles, "tile states")))
       return rc;
     JFC_HIP(hipMemcpy(d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_tile_inst, tile_inst.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -225,6 +240,7 @@ int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int 
 
 void graph_mark_inputs(jfc_graph *g, int inst) {
   g->caps_set[inst] = ~0;
+  g->mask_dirty = true;
   g->solved = false;
 }
 }  // namespace jfc
@@ -245,8 +261,8 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->stream = nullptr;
   o->initial_relabel = 0;
   o->stall_ratio = 0.99;
-  o->check_every = 2;
-  o->max_local_iters = 8;
+  o->check_every = 1;
+  o->max_local_iters = 4;
   o->relabel_work = 2.0;
 }
 
@@ -295,8 +311,10 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->cap, N * g->Kg, "residual capacities")) ||
       (rc = dalloc(&g->e, N, "excess")) || (rc = dalloc(&g->h, N, "heights")) ||
       (rc = dalloc(&g->lab, N, "labels")) || (rc = dalloc(&g->d_ctr, 8, "queues")) ||
+      (rc = dalloc(&g->mask0, N, "residual masks")) ||
+      (rc = dalloc(&g->rmask, N, "residual masks")) ||
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
-      (rc = dalloc(&g->d_total, 4, "counters")) ||
+      (rc = dalloc(&g->d_total, 5, "counters")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
     delete g;
     return rc;
@@ -429,6 +447,7 @@ extern "C" int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int3
   launch_zero_border(g->shape, g->inst[inst], g->c0 + (long long)arc * g->Ntot, arc, 0);
   JFC_HIP(hipDeviceSynchronize());
   g->caps_set[inst] |= 2 << arc;
+  g->mask_dirty = true;
   g->solved = false;
   return JFC_OK;
 }
@@ -518,29 +537,37 @@ struct Solver {
   // relabel / cut passes issued between two host checks
   static constexpr int kPassGroup = 4;
 
-  // Tile passes of a monotone search (relabel or cut) until no tile is dirty.
-  // The first pass visits every tile in jump-flooding distance order; later
-  // passes visit the dirty tiles, alternating direction (back-and-forth,
-  // QGCWidget.cpp:2221-2240).
-  int search(bool cut, int *passes, long long *tiles) {
+  enum SearchMode { kRelabel = 0, kCut = 1, kTag = 2 };
+
+  // Tile passes of a monotone search (relabel, cut or tag) until no tile is
+  // dirty.  The first pass visits every tile (relabel) or the seed tiles (cut,
+  // tag) in jump-flooding distance order; later passes visit the dirty tiles,
+  // alternating direction (back-and-forth, QGCWidget.cpp:2221-2240).
+  int search(int mode, int *passes, long long *tiles) {
     const Tiles T = g->tiles();
-    const Planes P = g->planes();
+    Planes P = g->planes();
+    if (mode == kTag) P.lab = g->tag;
     launch_tile_order(g->shape, T, g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist, g->d_hist,
                       g->nbins, g->d_order, g->max_tile_dim, nullptr, st);
     launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
     const int nwg = std::min(g->nwg, g->ntiles);
-    long long *total = g->d_total + (cut ? 2 : 1);
+    // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
+    long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
     long long before = 0;
     if (int rc = read_ll(total, &before)) return rc;
-    if (cut) {
-      // tiles without excess gain labels only through their halo, so the
+    auto flood = [&](const Queue &q) {
+      if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
+      else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
+      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
+    };
+    if (mode != kRelabel) {
+      // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
       launch_compact(g->d_order, g->ntiles, 0, g->d_seed, q, nullptr, total, st);
-      launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
+      flood(q);
     } else {
-      Queue q{g->d_order, g->d_ctr, g->slot, g->ntiles};
-      launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
+      flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles});
     }
     advance();
     *passes += 1;
@@ -550,8 +577,7 @@ struct Solver {
       for (int k = 0; k < kPassGroup; ++k, ++pass) {
         Queue qc = queue_from_list();
         launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, qc, nullptr, total, st);
-        if (cut) launch_reach(g->shape, nwg, T, P, qc, g->d_dirty, g->d_flagword, st);
-        else launch_relax(g->shape, nwg, T, P, qc, g->d_dirty, g->d_flag, st);
+        flood(qc);
         advance();
         *passes += 1;
       }
@@ -563,15 +589,24 @@ struct Solver {
     }
     long long after = 0;
     if (int rc = read_ll(total, &after)) return rc;
-    *tiles += after - before + (cut ? 0 : g->ntiles);
+    *tiles += after - before + (mode == kRelabel ? g->ntiles : 0);
     return JFC_OK;
+  }
+
+  // Tag post-process over the current labels; g->tag holds the foreground
+  // seed bytes on entry.
+  int tag_pass(int *passes, long long *tiles) {
+    launch_fill_i32(g->d_ctr, 8, 0, st);
+    g->slot = 0;
+    launch_tag_init(g->shape, g->tiles(), g->tag, g->d_seed, st);
+    return search(kTag, passes, tiles);
   }
 
   int global_relabel() {
     hipEvent_t a = tm.mark();
     launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
     long long tiles = 0;
-    int rc = search(false, &s.relabel_passes, &tiles);
+    int rc = search(kRelabel, &s.relabel_passes, &tiles);
     if (rc) return rc;
     s.relabel_tiles += tiles;  // k_relax left every tile's active flag
     gr_ev.push_back({a, tm.mark()});
@@ -585,7 +620,7 @@ struct Solver {
     launch_fill_i32(g->d_flagword, 1, 0, st);
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
     long long tiles = 0;
-    int rc = search(true, &s.cut_passes, &tiles);
+    int rc = search(kCut, &s.cut_passes, &tiles);
     if (rc) return rc;
     s.cut_tiles += tiles;
     cut_ev.push_back({a, tm.mark()});
@@ -606,11 +641,18 @@ struct Solver {
     hipEvent_t t0 = tm.mark();
     launch_fill_i32(g->d_ctr, 8, 0, st);
     launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
-    launch_fill_i32((int *)g->d_total, 8, 0, st);
+    launch_fill_i32((int *)g->d_total, 10, 0, st);
     launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
     g->slot = 0;
-    JFC_HIP(hipMemcpyAsync(g->cap, g->c0, (size_t)g->Ntot * g->Kg * sizeof(int),
+    // residual planes are copied from the inputs lazily, per touched tile
+    // (k_materialize); the residual-arc masks restart from the inputs'
+    if (g->mask_dirty) {
+      launch_build_mask(g->shape, T, P, g->mask0, st);
+      g->mask_dirty = false;
+    }
+    JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
+    launch_fill_i32(g->d_fresh, g->ntiles, 1, st);
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
     std::vector<long long> red((size_t)nI * RED_SLOTS);
     JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
@@ -640,6 +682,8 @@ struct Solver {
           const int n = g->color_off[c + 1] - g->color_off[c];
           launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, q,
                          g->d_sweep + sweep, g->d_total, st);
+          launch_materialize(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q,
+                             g->d_fresh, st);
           hipEvent_t a = o.profile ? tm.mark() : nullptr;
           launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
                       o.max_local_iters, g->d_total + 3, st);
@@ -770,6 +814,31 @@ extern "C" int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_de
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   JFC_HIP(hipMemcpy(labels, g->lab + g->inst[inst].node_off, g->inst_nodes(inst),
                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *tags,
+                            int on_device) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  if (!fg || !tags) return fail(JFC_ERR_INVALID, "null foreground or output");
+  if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  if (!g->tag && (rc = dalloc(&g->tag, (size_t)g->Ntot, "tag plane"))) return rc;
+  const long long n = g->inst_nodes(inst), o = g->inst[inst].node_off;
+  const hipMemcpyKind in = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind out = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  launch_fill_u8(g->tag, g->Ntot, 0, 0);
+  JFC_HIP(hipMemcpyAsync(g->tag + o, fg, (size_t)n, in, 0));
+  Solver S{};
+  S.g = g;
+  jfc_default_options(&S.o);
+  S.st = 0;
+  S.tm.pool = &g->ev_pool;
+  S.tm.st = 0;
+  int passes = 0;
+  long long tiles = 0;
+  if ((rc = S.tag_pass(&passes, &tiles))) return rc;
+  JFC_HIP(hipMemcpy(tags, g->tag + o, (size_t)n, out));
   return JFC_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
 
 namespace jfc {
 
@@ -36,6 +37,7 @@ struct Planes {
   const int *ct;     // input sink capacities
   unsigned char *lab;// canonical labels (1 = source side)
   long long stride;  // nodes per plane (all instances)
+  unsigned *rmask;   // bit d set iff cap[d] > 0 (kept exact by k_push)
 };
 
 struct Tiles {
@@ -64,6 +66,9 @@ struct Queue {
   int slot;
   int fixed_count;  // >= 0: consume list[0..fixed_count) instead of ctr count
 };
+
+// Sets the thread-local message returned by jfc_last_error().
+void set_last_error(const std::string &msg);
 
 // ---- launchers (all asynchronous on `st`) --------------------------------
 void launch_fill_i32(int *p, long long n, int v, hipStream_t st);
@@ -96,6 +101,10 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const Queue &q, int *sweep_cnt, long long *total,
                     hipStream_t st);
+void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
+                       hipStream_t st);
+void launch_materialize(int shape, int nwg, const Tiles &T, const Planes &P,
+                        const Queue &q, int *fresh, hipStream_t st);
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, int max_iters,
                  long long *iter_total, hipStream_t st);
@@ -105,6 +114,13 @@ void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st);
+// Tag post-process: init (tag plane holds foreground bytes on entry) and one
+// flood pass over the queued tiles.
+void launch_tag_init(int shape, const Tiles &T, unsigned char *tag,
+                     unsigned char *seed, hipStream_t st);
+void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
+                const Queue &q, unsigned char *dirty, const unsigned char *object,
+                hipStream_t st);
 void launch_deficit(int shape, const Tiles &T, const Planes &P, long long *red,
                     hipStream_t st);
 void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,

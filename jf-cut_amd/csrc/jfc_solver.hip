@@ -474,6 +474,41 @@ void launch_cut_init(int shape, const Tiles &T, const Planes &P,
   JFC_DISPATCH(shape, (k_cut_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, seed)));
 }
 
+// graphcut_init_tag (graphcut_compute_tag.cl:19-49): the tag plane arrives
+// holding the foreground seed bytes; it leaves as 0/1, and tiles with a seed
+// seed the jump flooding.
+template <class S>
+__global__ __launch_bounds__(256) void k_tag_init(Tiles T, unsigned char *tag,
+                                                  unsigned char *seed) {
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  __shared__ int s_w[8];
+  int par = 0;
+  for (int t = C.t0; t < C.t1; t += TU) {
+    TileNode nd[TU];
+    unsigned char f[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) f[u] = nd[u].valid ? tag[nd[u].v] : 0;
+    int bits = 0;
+#pragma unroll
+    for (int u = 0; u < TU; ++u)
+      if (nd[u].valid) {
+        tag[nd[u].v] = (unsigned char)(f[u] != 0);
+        bits |= (f[u] != 0) << u;
+      }
+    bits = block_or_bits(bits, s_w, par ^= 1);
+    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
+      seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
+  }
+}
+
+void launch_tag_init(int shape, const Tiles &T, unsigned char *tag, unsigned char *seed,
+                     hipStream_t st) {
+  JFC_DISPATCH(shape, (k_tag_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, tag, seed)));
+}
+
 // graphcut_count_node: a tile is active if one of its nodes has excess and a
 // finite height.
 template <class S>
@@ -714,20 +749,109 @@ struct QueueCursor {
     chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
   }
   // Next list index for the whole workgroup (idx/end are uniform); false
-  // once the list is drained.  Between two calls the caller executes at
-  // least one barrier, so s_idx is never overwritten while being read.
+  // once the list is drained.  The first run of every workgroup is static
+  // (group b takes [b*chunk, (b+1)*chunk)), so groups of a short or empty
+  // list leave without touching the shared head; later runs are taken from
+  // the head, which starts past the static runs.  Between two calls the
+  // caller executes at least one barrier, so s_idx is never overwritten while
+  // being read.
   __device__ __forceinline__ bool next(int &idx, int &end, int *s_idx) {
     if (idx + 1 < end) {
       ++idx;
       return true;
     }
-    if (threadIdx.x == 0) *s_idx = atomicAdd(head, chunk);
-    __syncthreads();
-    idx = *s_idx;
+    if (end == 0) {  // first call
+      idx = blockIdx.x * chunk;
+    } else {
+      if (threadIdx.x == 0) *s_idx = gridDim.x * chunk + atomicAdd(head, chunk);
+      __syncthreads();
+      idx = *s_idx;
+    }
     end = min(idx + chunk, count);
     return idx < count;
   }
 };
+
+// ---- residual-arc masks and lazy residual planes ----------------------------------
+//
+// rmask[v] bit d = (cap[d][v] > 0).  The relabel and cut searches only need
+// these bits, so they read 4 bytes per node instead of K planes.  mask0 is
+// the same for the input planes, built once per change of the inputs.
+template <class S>
+__global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned *mask0) {
+  const Lane<S> L(threadIdx.x);
+  const Chunk C(T.ntiles);
+  for (int t = C.t0; t < C.t1; ++t) {
+    const TileNode nd = tile_node<S>(T, t, C.t1, L);
+    if (!nd.valid) continue;
+    const int K = T.inst[nd.inst].K;
+    unsigned m = 0;
+#pragma unroll
+    for (int d = 0; d < S::KMAX; ++d)
+      if (d < K && P.c0[d * P.stride + nd.v] > 0) m |= 1u << d;
+    mask0[nd.v] = m;
+  }
+}
+
+void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
+                       hipStream_t st) {
+  JFC_DISPATCH(shape, (k_build_mask<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, mask0)));
+}
+
+// A solve starts with every tile "fresh": its residual planes equal the input
+// planes and live only in c0.  Before a push launch, each listed tile and
+// each neighbour it can flush into gets its planes copied c0 -> cap once
+// (one atomic exchange elects the copying group).  Tiles never pushed are
+// never copied.
+template <class S>
+__global__ __launch_bounds__(256) void k_materialize(Tiles T, Planes P, Queue q,
+                                                     int *fresh) {
+  __shared__ int s_take[S::D3 ? 7 : 9];
+  const Lane<S> L(threadIdx.x);
+  const int count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[2 * q.slot];
+  const long long st = P.stride;
+  for (int i = blockIdx.x; i < count; i += gridDim.x) {
+    const int t = q.list[i];
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    constexpr int NC = S::D3 ? 7 : 9;
+    if (threadIdx.x < NC) {
+      int dx = 0, dy = 0, dz = 0;
+      if (S::D3) {
+        if (threadIdx.x > 0) dir_of<true>(threadIdx.x - 1, dx, dy, dz);
+      } else {
+        dx = (int)(threadIdx.x % 3) - 1;
+        dy = (int)(threadIdx.x / 3) - 1;
+      }
+      const int X = r.ltx + dx, Y = r.lty + dy, Z = r.ltz + dz;
+      int take = -1;
+      if (X >= 0 && X < I.tx && Y >= 0 && Y < I.ty && Z >= 0 && Z < I.tz) {
+        const int n = I.tile_off + X + I.tx * (Y + I.ty * Z);
+        if (atomicExch(&fresh[n], 0)) take = n;
+      }
+      s_take[threadIdx.x] = take;
+    }
+    __syncthreads();
+    for (int c = 0; c < NC; ++c) {
+      const int n = s_take[c];
+      if (n < 0) continue;
+      const TileRef rn = tile_ref<S>(T, n);
+      const int gx = rn.ox + L.lx, gy = rn.oy + L.ly, gz = rn.oz + L.lz;
+      if (gx < I.nx && gy < I.ny && gz < I.nz) {
+        const long long v = node_index(I, gx, gy, gz);
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d)
+          if (d < I.K) P.cap[d * st + v] = P.c0[d * st + v];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_materialize(int shape, int nwg, const Tiles &T, const Planes &P,
+                        const Queue &q, int *fresh, hipStream_t st) {
+  JFC_DISPATCH(shape, (k_materialize<S><<<nwg, 256, 0, st>>>(T, P, q, fresh)));
+}
 
 // ---- block-wise push-relabel -------------------------------------------------------
 //
@@ -856,9 +980,16 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
       const int e1 = s_e[L.me], h1 = s_h[L.me];
       if (e1 != e0) P.e[gv] = e1;
       if (h1 != h0) P.h[gv] = h1;
+      unsigned m0 = 0, m1 = 0;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
-        if (d < K && s_c[d][L.me] != c0r[d]) P.cap[d * st + gv] = s_c[d][L.me];
+        if (d < K) {
+          const int c1 = s_c[d][L.me];
+          if (c1 != c0r[d]) P.cap[d * st + gv] = c1;
+          m0 |= (unsigned)(c0r[d] > 0) << d;
+          m1 |= (unsigned)(c1 > 0) << d;
+        }
+      if (m1 != m0) P.rmask[gv] = m1;
       act = e1 > 0 && h1 < HINF;
     }
     // flush flow that crossed into the halo: excess and the reverse residual
@@ -876,12 +1007,19 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
         atomicAdd(&P.e[g], f);
         flag[tile_of<S>(I, X, Y, Z)] = 1;
       }
+      unsigned bits = 0;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
         if (d < K) {
           const int dc = s_c[d][p];
-          if (dc != 0) P.cap[d * st + g] += dc;
+          if (dc != 0) {
+            P.cap[d * st + g] += dc;
+            bits |= 1u << d;
+          }
         }
+      // the crossing arcs' reverse residuals are now > 0; several tiles of
+      // this colour class may flush into one halo node, hence the atomic
+      if (bits) atomicOr(&P.rmask[g], bits);
     }
     if (__syncthreads_or(act) && tid == 0) flag[t] = 1;
   }
@@ -908,7 +1046,6 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   const QueueCursor Q(q);
-  const long long st = P.stride;
   QueueCursor Qc = Q;
   int idx = 0, end = 0;
   while (Qc.next(idx, end, &s_idx)) {
@@ -925,9 +1062,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
       gv = node_index(I, gx, gy, gz);
       h0 = P.h[gv];
       e0 = P.e[gv];
-#pragma unroll
-      for (int d = 0; d < S::KMAX; ++d)
-        if (d < K && P.cap[d * st + gv] > 0) mask |= 1u << d;
+      mask = P.rmask[gv] & ((1u << K) - 1u);
     }
     s_h[L.me] = h0;
     for (int p = tid; p < S::PADN; p += S::NT) {
@@ -1003,36 +1138,45 @@ void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
 // and mark the neighbour tile for the next pass.  A labelled node with a
 // deficit means an augmenting path still exists: *cert is set
 // (graphcut_compute_cut.cl:135-139).
-template <class S>
+//
+// TAG: the same flood for the tag post-process (graphcut_compute_tag.cl:51-140,
+// :142-242): P.lab is the tag plane, a tagged node tags its axis neighbours
+// (4 in 2D, 6 in 3D) that are on the source side of the cut (`object`).
+// LDS codes: 0 open, 1 labelled on entry, 2 labelled here, 4 closed.
+template <class S, bool TAG>
 __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
-                                               unsigned char *dirty, int *cert) {
+                                               unsigned char *dirty, int *cert,
+                                               const unsigned char *object) {
   __shared__ unsigned char s_l[S::PADN];
   __shared__ int s_idx;
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   const QueueCursor Q(q);
-  const long long st = P.stride;
   QueueCursor Qc = Q;
+  constexpr unsigned kAxis = S::D3 ? 0x3fu : 0xfu;
   int idx = 0, end = 0;
   while (Qc.next(idx, end, &s_idx)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
-    const int K = I.K;
+    const int K = TAG ? (S::D3 ? 6 : 4) : I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
     long long gv = 0;
-    unsigned char l0 = 0;
+    unsigned char l0 = TAG ? 4 : 0;
     unsigned mask = 0;
     bool deficit = false;
     if (valid) {
       gv = node_index(I, gx, gy, gz);
-      l0 = P.lab[gv];
-      deficit = P.e[gv] < 0;
-#pragma unroll
-      for (int d = 0; d < S::KMAX; ++d)
-        if (d < K && P.cap[d * st + gv] > 0) mask |= 1u << d;
-      if (l0 && deficit) *cert = 1;
+      if (TAG) {
+        l0 = P.lab[gv] ? 1 : object[gv] ? 0 : 4;
+        mask = kAxis;
+      } else {
+        l0 = P.lab[gv];
+        deficit = P.e[gv] < 0;
+        mask = P.rmask[gv] & ((1u << K) - 1u);
+        if (l0 && deficit) *cert = 1;
+      }
     }
     s_l[L.me] = l0;
     for (int p = tid; p < S::PADN; p += S::NT) {
@@ -1043,8 +1187,11 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
       unsigned char lv = 1;
       if (halo_needed<S>(nout, K)) {
         const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
-          lv = P.lab[node_index(I, X, Y, Z)];
+        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz) {
+          const long long hv = node_index(I, X, Y, Z);
+          lv = P.lab[hv];
+          if (TAG) lv = lv ? 1 : object[hv] ? 0 : 4;
+        }
       }
       s_l[p] = lv;
     }
@@ -1052,7 +1199,8 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
     int any;
     do {
       int ch = 0;
-      if (mask && s_l[L.me]) {
+      const unsigned me = s_l[L.me];
+      if (mask && (me == 1u || me == 2u)) {
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d)
           if (mask & (1u << d)) {
@@ -1067,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
     } while (any);
     if (valid && s_l[L.me] == 2) {
       P.lab[gv] = 1;
-      if (deficit) *cert = 1;
+      if (!TAG && deficit) *cert = 1;
     }
     for (int p = tid; p < S::PADN; p += S::NT) {
       if (s_l[p] != 2) continue;
@@ -1086,7 +1234,13 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st) {
-  JFC_DISPATCH(shape, (k_reach<S><<<nwg, 256, 0, st>>>(T, P, q, dirty, cert)));
+  JFC_DISPATCH(shape, (k_reach<S, false><<<nwg, 256, 0, st>>>(T, P, q, dirty, cert, nullptr)));
+}
+
+void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
+                const Queue &q, unsigned char *dirty, const unsigned char *object,
+                hipStream_t st) {
+  JFC_DISPATCH(shape, (k_reach<S, true><<<nwg, 256, 0, st>>>(T, P, q, dirty, nullptr, object)));
 }
 
 }  // namespace jfc

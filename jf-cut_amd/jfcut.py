@@ -83,7 +83,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_get_terminal_caps", "jfc_get_neighbor_caps", "jfc_debug_tiles",
            "jfc_debug_tile_order", "jfc_debug_sweep_counts", "jfc_backward_all",
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
-           "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write"]
+           "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
+           "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags"]
 
 
 def lib():
@@ -131,8 +132,12 @@ def lib():
         "jfc_mfi_write": (I, [ctypes.c_char_p, I, I, I, I, P, P, P, I64]),
         "jfc_raw_read": (I, [ctypes.c_char_p, I, I, I, P, P]),
         "jfc_raw_write": (I, [ctypes.c_char_p, I, I, I, P, P]),
-        "jfc_cut_read": (I, [ctypes.c_char_p, I64, P, P, P]),
-        "jfc_cut_write": (I, [ctypes.c_char_p, I64, P, P, P]),
+        "jfc_cut_read": (I, [ctypes.c_char_p, I64, P, P, P, P]),
+        "jfc_cut_write": (I, [ctypes.c_char_p, I64, P, P, P, P]),
+        "jfc_node_read": (I, [ctypes.c_char_p, I, I, I, I, I, I, P, P, P]),
+        "jfc_node_write": (I, [ctypes.c_char_p, I, I, I, I, I, I, P, P, P]),
+        "jfc_mfi_export": (I, [ctypes.c_char_p, I, I, I, P, P, P, P, P, P, I64]),
+        "jfc_get_tags": (I, [P, I, P, P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -283,6 +288,22 @@ class GridGraph:
         lab = np.zeros(self.nodes(inst), np.uint8)
         _check(lib().jfc_get_labels(self._h, inst, lab.ctypes.data_as(ctypes.c_void_p), 0))
         return lab
+
+    def tags(self, fg, inst=0, out=None):
+        """Tag post-process of the solved cut (graphcut_compute_tag.cl): 1 for
+        foreground seeds and source-side nodes axis-connected to one through
+        source-side nodes.  fg: numpy array (host) or GPU tensor (then `out`
+        must be a GPU uint8 tensor too)."""
+        if hasattr(fg, "data_ptr"):
+            _check(lib().jfc_get_tags(self._h, inst, ctypes.c_void_p(fg.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr()), 1))
+            return out
+        f = np.ascontiguousarray(fg, np.uint8).ravel()
+        if f.size != self.nodes(inst):
+            raise ValueError("fg must hold one byte per node of the instance")
+        t = np.zeros(f.size, np.uint8)
+        _check(lib().jfc_get_tags(self._h, inst, _vp(f), _vp(t), 0))
+        return t
 
     def size(self):
         """(total nodes over all instances, largest connectivity)."""
@@ -523,18 +544,62 @@ def write_raw_graph(prefix, ef, caps):
 
 
 def read_cut(path, n):
-    """A .cut file (QGCW:2971-2983): (fg, bg, object) as uint8 0/1 arrays."""
-    out = [np.zeros(n, np.uint8) for _ in range(3)]
+    """A .cut file (QGCW:2971-2983): (fg, bg, object, tag) as uint8 0/1 arrays."""
+    out = [np.zeros(n, np.uint8) for _ in range(4)]
     _check(lib().jfc_cut_read(os.fsencode(path), n, *[_vp(a) for a in out]))
     return tuple(out)
 
 
-def write_cut(path, fg=None, bg=None, obj=None):
+def write_cut(path, fg=None, bg=None, obj=None, tag=None):
     arrs = [None if a is None else np.ascontiguousarray(a, np.uint8).ravel()
-            for a in (fg, bg, obj)]
+            for a in (fg, bg, obj, tag)]
     n = next(a.size for a in arrs if a is not None)
     _check(lib().jfc_cut_write(os.fsencode(path), n,
                                *[None if a is None else _vp(a) for a in arrs]))
+
+
+def _block(nz, block):
+    if block is not None:
+        return block
+    return (8, 8, 4) if nz > 1 else (16, 16, 1)
+
+
+def read_node(path, nx, ny, nz=1, block=None):
+    """A .node file (QGCW:2642-2717): (ef, height, caps) on the unpadded grid.
+    block = tile shape the file was padded to (default 16x16 / 8x8x4)."""
+    bx, by, bz = _block(nz, block)
+    K = 6 if nz > 1 else 4
+    ef = np.zeros((nz, ny, nx), np.int32)
+    h = np.zeros_like(ef)
+    caps = np.zeros((K, nz, ny, nx), np.int32)
+    _check(lib().jfc_node_read(os.fsencode(path), nx, ny, nz, bx, by, bz, _vp(ef), _vp(h),
+                               _vp(caps)))
+    return ef, h, caps
+
+
+def write_node(path, ef, caps, height=None, block=None):
+    ef = np.ascontiguousarray(ef, np.int32)
+    caps = np.ascontiguousarray(caps, np.int32)
+    nz, ny, nx = ef.shape if ef.ndim == 3 else (1,) + ef.shape
+    bx, by, bz = _block(nz, block)
+    hp = None
+    if height is not None:
+        height = np.ascontiguousarray(height, np.int32)
+        hp = _vp(height)
+    _check(lib().jfc_node_write(os.fsencode(path), nx, ny, nz, bx, by, bz, _vp(ef), hp,
+                                _vp(caps)))
+
+
+def export_mfi(path, ef, caps, fg=None, bg=None, obj=None, tag=None, maxflow=0):
+    """The widget's MFI export (saveMFI, QGCW:3339-3473) from folded ef and
+    4 (2D) / 6 (3D) direction planes plus optional cut channels."""
+    ef = np.ascontiguousarray(ef, np.int32)
+    caps = np.ascontiguousarray(caps, np.int32)
+    nz, ny, nx = ef.shape if ef.ndim == 3 else (1,) + ef.shape
+    ch = [None if a is None else np.ascontiguousarray(a, np.uint8).ravel()
+          for a in (fg, bg, obj, tag)]
+    _check(lib().jfc_mfi_export(os.fsencode(path), nx, ny, nz, _vp(ef), _vp(caps),
+                                *[None if a is None else _vp(a) for a in ch], int(maxflow)))
 
 
 # ---- differentiable min cut -------------------------------------------------------

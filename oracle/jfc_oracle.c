@@ -129,6 +129,38 @@ int orc_canonical_cut(int nx, int ny, int nz, int conn, const int32_t *e,
   return ok;
 }
 
+/* graphcut_init_tag + graphcut_compute_tag (graphcut_compute_tag.cl:19-49,
+ * 51-140 2D, :142-242 3D): foreground seeds are tagged; an object node with a
+ * tagged axis neighbour (4 in 2D whatever the graph's connectivity, 6 in 3D)
+ * becomes tagged, to the fixpoint.  Breadth-first over a queue here. */
+int orc_tag(int nx, int ny, int nz, const uint8_t *fg, const uint8_t *object,
+            uint8_t *tag) {
+  grid_t g;
+  if (grid_init(&g, nx, ny, nz, nz > 1 ? 6 : 4)) return -1;
+  const int64_t N = g.N;
+  int64_t *queue = (int64_t *)malloc((size_t)N * sizeof(int64_t));
+  if (!queue) { free(g.mask); return -1; }
+  int64_t head = 0, tail = 0;
+  for (int64_t v = 0; v < N; ++v) {
+    tag[v] = fg[v] != 0;
+    if (tag[v]) queue[tail++] = v;
+  }
+  while (head < tail) {
+    const int64_t u = queue[head++];
+    for (int d = 0; d < g.K; ++d) {
+      if (!(g.mask[u] & (1u << d))) continue;
+      const int64_t v = u + g.off[d];
+      if (!tag[v] && object[v]) {
+        tag[v] = 1;
+        queue[tail++] = v;
+      }
+    }
+  }
+  free(queue);
+  free(g.mask);
+  return 0;
+}
+
 int64_t orc_energy(int nx, int ny, int nz, int conn, const int32_t *cs,
                    const int32_t *ct, const int32_t *caps,
                    const uint8_t *labels) {

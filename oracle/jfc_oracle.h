@@ -19,6 +19,8 @@
  *                        the residual graph (graphcut_compute_cut.cl:19-140),
  *                        plus the convergence certificate (no such node has a
  *                        deficit, graphcut_compute_cut.cl:135-139).
+ *   orc_tag              foreground-connected part of the object
+ *                        (graphcut_compute_tag.cl:19-242).
  *   orc_energy           Sum_T cs + Sum_S ct + Sum_{u in S, v in T} c(u->v) on
  *                        the initial capacities (QGCWidget.cpp:2452-2520).
  *   orc_kmeans_*         integer k-means data term (restating the intent of
@@ -56,6 +58,12 @@ int64_t orc_maxflow(int nx, int ny, int nz, int conn,
                     const int32_t *cs, const int32_t *ct, const int32_t *caps,
                     int32_t *e_out, int32_t *res_out, uint8_t *labels_out,
                     int *certificate_ok);
+
+/* Tag post-process: tag[v] = 1 iff v is a foreground seed or an object node
+ * joined to one through axis-adjacent (4 in 2D, 6 in 3D) object nodes.
+ * Returns 0, or -1 on allocation failure. */
+int orc_tag(int nx, int ny, int nz, const uint8_t *fg, const uint8_t *object,
+            uint8_t *tag);
 
 /* Canonical cut of a residual state: labels[v] = 1 iff v is reachable from a
  * node with e > 0 along arcs with residual > 0.  Returns 1 iff no reached node

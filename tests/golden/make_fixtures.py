@@ -8,7 +8,10 @@ where the JF-Cut reference tree is mounted at /root/reference).
    canonical labels: flower's from the .cut object channel (byte 2), person's and
    sponge's from BK303 (the reference's CPU path, oracle/_ref) with free nodes on
    the sink side.
-2. c1_goldens.json: config-1 instances (512x512, 4-connected) built from the
+2. ref_flower_cut.npz: the remaining channels of flower's .cut file
+   (foreground seeds byte 0, background seeds byte 1, tag byte 3), the output
+   of the reference's tag post-process (graphcut_compute_tag.cl) for that cut.
+3. c1_goldens.json: config-1 instances (512x512, 4-connected) built from the
    synthetic generator of libjfcut plus the oracle's integer data term, solved by
    BK303: flow, |S|, sha256 of labels and of the capacity planes.
 
@@ -68,6 +71,16 @@ def ref_fixtures():
         print(name, nx, ny, flow, int(lab.sum()), source)
 
 
+def flower_cut_channels():
+    pre = f"{REF}/flower/normalized_flower.raw"
+    cut = np.fromfile(pre + ".cut", dtype=np.uint8).reshape(-1, 4)
+    assert set(np.unique(cut)) <= {0, 127}
+    ch = {k: np.packbits(cut[:, i] != 0) for k, i in (("fg", 0), ("bg", 1), ("tag", 3))}
+    np.savez_compressed(os.path.join(HERE, "ref_flower_cut.npz"), n=np.int64(len(cut)), **ch)
+    print("flower cut channels", {k: int((cut[:, i] != 0).sum()) for k, i in
+                                  (("fg", 0), ("bg", 1), ("tag", 3))})
+
+
 def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
@@ -94,5 +107,10 @@ def c1_goldens(seeds=(0, 1, 2, 3)):
 
 
 if __name__ == "__main__":
-    ref_fixtures()
-    c1_goldens()
+    parts = sys.argv[1:] or ["ref", "flower_cut", "c1"]
+    if "ref" in parts:
+        ref_fixtures()
+    if "flower_cut" in parts:
+        flower_cut_channels()
+    if "c1" in parts:
+        c1_goldens()

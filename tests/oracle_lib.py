@@ -44,6 +44,8 @@ def orc():
         L.orc_pair_thresholds.argtypes = [ctypes.c_double, I64, I64, P]
         L.orc_pairwise.restype = None
         L.orc_pairwise.argtypes = [I, I, I, I, P, I, ctypes.c_double, ctypes.c_double, I, P]
+        L.orc_tag.restype = I
+        L.orc_tag.argtypes = [I, I, I, P, P, P]
         _orc = L
     return _orc
 
@@ -89,6 +91,15 @@ def canonical_cut(nx, ny, nz, conn, e, res):
     ok = orc().orc_canonical_cut(nx, ny, nz, conn, vp(_i32(e)), vp(_i32(res).reshape(conn, N)),
                                  vp(lab))
     return lab, ok
+
+
+def tag(nx, ny, nz, fg, obj):
+    """Oracle tag post-process (graphcut_compute_tag.cl semantics)."""
+    fg = np.ascontiguousarray(fg, np.uint8).ravel()
+    obj = np.ascontiguousarray(obj, np.uint8).ravel()
+    out = np.zeros(fg.size, np.uint8)
+    assert orc().orc_tag(nx, ny, nz, vp(fg), vp(obj), vp(out)) == 0
+    return out
 
 
 def energy(nx, ny, nz, conn, cs, ct, caps, labels):

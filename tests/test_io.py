@@ -152,13 +152,12 @@ def test_raw_graph_reference_files(tmp_path, jfcut_mod, name):
     jfcut_mod.write_raw_graph(out, ef2, caps2)
     for ext in (".ef", ".c1", ".c2"):
         assert open(out + ext, "rb").read() == open(pre + ext, "rb").read(), ext
-    fg, bg, obj = jfcut_mod.read_cut(pre + ".cut", nx * ny)
+    fg, bg, obj, tag = jfcut_mod.read_cut(pre + ".cut", nx * ny)
     if name == "flower":
         assert np.array_equal(obj, lab)
-    jfcut_mod.write_cut(out + ".cut", fg, bg, obj)
-    mine = np.fromfile(out + ".cut", np.uint8).reshape(-1, 4)
-    ref = np.fromfile(pre + ".cut", np.uint8).reshape(-1, 4)
-    assert np.array_equal(mine[:, :3], ref[:, :3])
+        assert np.array_equal(tag, ol.tag(nx, ny, 1, fg, obj))
+    jfcut_mod.write_cut(out + ".cut", fg, bg, obj, tag)
+    assert open(out + ".cut", "rb").read() == open(pre + ".cut", "rb").read()
 
 
 def test_raw_graph_3d_and_limits(tmp_path, jfcut_mod):
@@ -189,11 +188,103 @@ def test_cut_roundtrip(tmp_path, jfcut_mod):
     fg, bg, obj = (rng.random(n) < 0.3).astype(np.uint8), (rng.random(n) < 0.3).astype(
         np.uint8), (rng.random(n) < 0.5).astype(np.uint8)
     p = str(tmp_path / "x.cut")
-    jfcut_mod.write_cut(p, fg, bg, obj)
+    tag = obj & (rng.random(n) < 0.5).astype(np.uint8)
+    jfcut_mod.write_cut(p, fg, bg, obj, tag)
     raw = np.fromfile(p, np.uint8).reshape(-1, 4)
-    assert set(np.unique(raw[:, :3])) <= {0, 127} and not raw[:, 3].any()
-    f2, b2, o2 = jfcut_mod.read_cut(p, n)
+    assert set(np.unique(raw)) <= {0, 127}
+    for k, a in enumerate((fg, bg, obj, tag)):
+        assert np.array_equal(raw[:, k] != 0, a.astype(bool)), k
+    f2, b2, o2, t2 = jfcut_mod.read_cut(p, n)
     assert np.array_equal(f2, fg) and np.array_equal(b2, bg) and np.array_equal(o2, obj)
+    assert np.array_equal(t2, tag)
     jfcut_mod.write_cut(p, obj=obj)
-    f3, b3, o3 = jfcut_mod.read_cut(p, n)
-    assert not f3.any() and not b3.any() and np.array_equal(o3, obj)
+    f3, b3, o3, t3 = jfcut_mod.read_cut(p, n)
+    assert not f3.any() and not b3.any() and not t3.any() and np.array_equal(o3, obj)
+
+
+# cl_node_2d / cl_node_3d records (QGCSetting.h:31-43) as numpy record types
+NODE2D = np.dtype([("height", "<i4"), ("ef", "<i4"), ("cap", "<u2", 4)])
+NODE3D = np.dtype([("height", "<i4"), ("ef", "<i4"), ("cap", "u1", 8)])
+
+
+@pytest.mark.parametrize("name", ["flower", "sponge"])
+def test_node_file_2d(tmp_path, jfcut_mod, name):
+    """Fixture -> .node (padded to 16x16 tiles) -> reader: the loader of
+    QGCW:2653-2665 maps record cap[0..3] to c1.x, c1.y, c2.x, c2.y, i.e.
+    +x, +y, -y, -x; the graph keeps its published flow."""
+    nx, ny, ef, cs, ct, caps, flow, lab = load_ref_fixture(name)
+    p = str(tmp_path / f"{name}.node")
+    h = np.arange(nx * ny, dtype=np.int32).reshape(ny, nx) % 1000
+    jfcut_mod.write_node(p, ef.reshape(ny, nx), caps.reshape(4, ny, nx), height=h)
+    px, py = -(-nx // 16) * 16, -(-ny // 16) * 16
+    rec = np.fromfile(p, NODE2D)
+    assert rec.size == px * py
+    rec = rec.reshape(py, px)
+    assert not rec[ny:].view(np.uint8).any() and not rec[:, nx:].view(np.uint8).any()
+    body = rec[:ny, :nx]
+    assert np.array_equal(body["ef"].ravel(), ef) and np.array_equal(body["height"], h)
+    c = caps.reshape(4, ny, nx)
+    for k, d in enumerate([0, 2, 3, 1]):
+        assert np.array_equal(body["cap"][..., k], c[d]), k
+    ef2, h2, caps2 = jfcut_mod.read_node(p, nx, ny)
+    assert np.array_equal(ef2.ravel(), ef) and np.array_equal(h2[0], h)
+    assert np.array_equal(caps2.reshape(4, -1), caps)
+    assert ol.maxflow(nx, ny, 1, 4, np.maximum(ef2, 0).ravel(), np.maximum(-ef2, 0).ravel(),
+                      caps2.reshape(4, -1))[0] == flow
+
+
+def test_node_file_3d(tmp_path, jfcut_mod):
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 13, 10, 6
+    ef = rng.integers(-500, 500, (nz, ny, nx)).astype(np.int32)
+    caps = rng.integers(0, 256, (6, nz, ny, nx)).astype(np.int32)
+    p = str(tmp_path / "v.node")
+    jfcut_mod.write_node(p, ef, caps)
+    rec = np.fromfile(p, NODE3D).reshape(8, 16, 16)  # padded to 8x8x4 tiles
+    body = rec[:nz, :ny, :nx]
+    assert np.array_equal(body["ef"], ef) and not body["height"].any()
+    # QGCW:2697-2712: cap[0..2] -> c1 (+x, +y, +z), cap[3..5] -> c2 (-z, -y, -x)
+    for k, d in enumerate([0, 2, 4, 5, 3, 1]):
+        assert np.array_equal(body["cap"][..., k], caps[d]), k
+    assert not body["cap"][..., 6:].any()
+    ef2, h2, caps2 = jfcut_mod.read_node(p, nx, ny, nz)
+    assert np.array_equal(ef2, ef) and np.array_equal(caps2, caps) and not h2.any()
+    # other tile padding
+    jfcut_mod.write_node(p, ef, caps, block=(4, 4, 2))
+    assert os.path.getsize(p) == 16 * 16 * 12 * 6
+    ef3, _, caps3 = jfcut_mod.read_node(p, nx, ny, nz, block=(4, 4, 2))
+    assert np.array_equal(ef3, ef) and np.array_equal(caps3, caps)
+    with pytest.raises(jfcut_mod.JFCError):  # file shorter than the padded grid
+        jfcut_mod.read_node(p, nx, ny, nz)
+
+
+def test_mfi_export_like_widget(tmp_path, jfcut_mod):
+    """saveMFI (QGCW:3339-3473) on flower: uint32 terminals, uint16 arcs,
+    symmetric, active_count = #(ef >= 0), mask codes 1/2/3/4 from the cut
+    channels; the reference reader and BK303 construction give the published
+    flow."""
+    L = mfi_ref()
+    nx, ny, ef, cs, ct, caps, flow, lab = load_ref_fixture("flower")
+    n = nx * ny
+    rng = np.random.default_rng(2)
+    fg = (rng.random(n) < 0.05).astype(np.uint8)
+    bg = (rng.random(n) < 0.05).astype(np.uint8)
+    tag = (rng.random(n) < 0.5).astype(np.uint8)
+    path = str(tmp_path / "flower.mfi")
+    jfcut_mod.export_mfi(path, ef.reshape(ny, nx), caps.reshape(4, ny, nx), fg=fg, bg=bg,
+                         obj=lab, tag=tag, maxflow=flow)
+    hdr, mf, rcs, rct, rcaps = _ref_read(L, path, n, 4)
+    assert list(hdr) == [2, 4, nx, ny, 1, 1, int((ef >= 0).sum()), 2, 1]
+    assert mf == flow
+    assert np.array_equal(rcs, np.maximum(ef, 0)) and np.array_equal(rct, np.maximum(-ef, 0))
+    for a in range(4):
+        assert np.array_equal(rcaps[a], caps[MFI2D[a]])
+    mask = np.fromfile(path, np.uint8)[-n:]
+    want = np.where(fg == 1, 1, np.where(bg == 1, 2, np.where(lab == 1, 3,
+                                                              np.where(tag == 1, 4, 0))))
+    assert np.array_equal(mask, want)
+    assert L.mfi_ref_bk303(os.fsencode(path)) == flow
+    big = caps.reshape(4, ny, nx).copy()
+    big[0, 0, 0] = 65536
+    with pytest.raises(jfcut_mod.JFCError, match="does not fit"):
+        jfcut_mod.export_mfi(path, ef.reshape(ny, nx), big)

@@ -271,3 +271,38 @@ def test_c1_goldens_cpu(jfcut_mod):
         if ol.have_bk():
             fb, lb = ol.bk303(g["W"], g["H"], 1, 4, cs, ct, caps)
             assert fb == f and np.array_equal(lb, l)
+
+
+def load_flower_cut():
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "ref_flower_cut.npz"))
+    n = int(z["n"])
+    return {k: np.unpackbits(z[k])[:n] for k in ("fg", "bg", "tag")}
+
+
+def test_tag_oracle_flower_golden():
+    """The tag channel of flower's .cut is the reference's tag post-process of
+    its foreground seeds and object channel."""
+    nx, ny, ef, cs, ct, caps, flow, lab = load_ref_fixture("flower")
+    ch = load_flower_cut()
+    assert np.array_equal(ol.tag(nx, ny, 1, ch["fg"], lab), ch["tag"])
+    assert ch["tag"].sum() == 52721
+
+
+@pytest.mark.parametrize("shape", [(41, 37, 1), (13, 11, 9)])
+def test_tag_oracle_vs_connected_components(shape):
+    """Closed form: connected components (axis adjacency) of object | fg that
+    hold a foreground seed, intersected with object | fg."""
+    from scipy import ndimage
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx)
+    for trial in range(5):
+        obj = (rng.random((nz, ny, nx)) < 0.55).astype(np.uint8)
+        fg = (rng.random((nz, ny, nx)) < 0.01).astype(np.uint8)
+        passable = (obj | fg).astype(bool)
+        st = ndimage.generate_binary_structure(3 if nz > 1 else 2, 1)
+        comp, _ = ndimage.label(passable if nz > 1 else passable[0], structure=st)
+        comp = comp.reshape(nz, ny, nx)
+        keep = np.unique(comp[fg.astype(bool)])
+        want = (np.isin(comp, keep[keep > 0]) & passable).astype(np.uint8)
+        assert np.array_equal(ol.tag(nx, ny, nz, fg, obj), want.ravel())
