@@ -109,6 +109,7 @@ struct jfc_graph {
   int *d_list[2] = {nullptr, nullptr};
   int *d_ctr = nullptr;
   unsigned char *d_flag = nullptr, *d_dirty = nullptr, *d_seed = nullptr;
+  unsigned char *d_mat = nullptr;  // per tile: residual planes written this solve
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
   long long *d_red = nullptr, *d_total = nullptr;
@@ -118,7 +119,6 @@ struct jfc_graph {
   int *d_flagword = nullptr;  // [0] certificate, [1] validation
   unsigned *mask0 = nullptr;  // residual-arc bits of the input planes
   unsigned *rmask = nullptr;  // residual-arc bits of the current residual
-  int *d_fresh = nullptr;     // per tile: residual planes not yet copied from c0
   bool mask_dirty = true;     // inputs changed since mask0 was built
   int slot = 0;
   bool solved = false;
@@ -131,7 +131,7 @@ struct jfc_graph {
   void release_tiles() {
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
-    dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_fresh);
+    dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
   }
   void release() {
     release_tiles();
@@ -211,7 +211,7 @@ struct jfc_graph {
         (rc = dalloc(&d_tdist, ntiles, "tile distance")) ||
         (rc = dalloc(&d_hist, nbins, "tile histogram")) ||
         (rc = dalloc(&d_order, ntiles, "tile order")) ||
-        (rc = dalloc(&d_fresh, ntiles, "tile states")))
+        (rc = dalloc(&d_mat, ntiles, "tile states")))
       return rc;
     JFC_HIP(hipMemcpy(d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_tile_inst, tile_inst.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -310,7 +310,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->c0, N * g->Kg, "neighbour capacities")) ||
       (rc = dalloc(&g->cap, N * g->Kg, "residual capacities")) ||
       (rc = dalloc(&g->e, N, "excess")) || (rc = dalloc(&g->h, N, "heights")) ||
-      (rc = dalloc(&g->lab, N, "labels")) || (rc = dalloc(&g->d_ctr, 8, "queues")) ||
+      (rc = dalloc(&g->lab, N, "labels")) || (rc = dalloc(&g->d_ctr, 2 * QSLOT, "queues")) ||
       (rc = dalloc(&g->mask0, N, "residual masks")) ||
       (rc = dalloc(&g->rmask, N, "residual masks")) ||
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
@@ -583,7 +583,7 @@ struct Solver {
       }
       int cnt = 0;
       // the last compaction's slot is the one before the current
-      if (int rc = read_int(g->d_ctr + 2 * (g->slot ^ 1), &cnt)) return rc;
+      if (int rc = read_int(g->d_ctr + QSLOT * (g->slot ^ 1), &cnt)) return rc;
       if (cnt == 0) break;
       if (pass > 1000000) return fail(JFC_ERR_STATE, "search did not settle");
     }
@@ -596,7 +596,7 @@ struct Solver {
   // Tag post-process over the current labels; g->tag holds the foreground
   // seed bytes on entry.
   int tag_pass(int *passes, long long *tiles) {
-    launch_fill_i32(g->d_ctr, 8, 0, st);
+    launch_fill_i32(g->d_ctr, 2 * QSLOT, 0, st);
     g->slot = 0;
     launch_tag_init(g->shape, g->tiles(), g->tag, g->d_seed, st);
     return search(kTag, passes, tiles);
@@ -639,20 +639,20 @@ struct Solver {
       g->sweep_cap = o.max_sweeps + 1;
     }
     hipEvent_t t0 = tm.mark();
-    launch_fill_i32(g->d_ctr, 8, 0, st);
+    launch_fill_i32(g->d_ctr, 2 * QSLOT, 0, st);
     launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
     launch_fill_i32((int *)g->d_total, 10, 0, st);
     launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
     g->slot = 0;
-    // residual planes are copied from the inputs lazily, per touched tile
-    // (k_materialize); the residual-arc masks restart from the inputs'
+    // the residual-arc masks restart from the inputs' with every plane
+    // marked fresh: k_push reads fresh planes from c0 and writes them to cap
     if (g->mask_dirty) {
       launch_build_mask(g->shape, T, P, g->mask0, st);
       g->mask_dirty = false;
     }
     JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
-    launch_fill_i32(g->d_fresh, g->ntiles, 1, st);
+    launch_fill_u8(g->d_mat, g->ntiles, 0, st);
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
     std::vector<long long> red((size_t)nI * RED_SLOTS);
     JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
@@ -682,11 +682,9 @@ struct Solver {
           const int n = g->color_off[c + 1] - g->color_off[c];
           launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, q,
                          g->d_sweep + sweep, g->d_total, st);
-          launch_materialize(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q,
-                             g->d_fresh, st);
           hipEvent_t a = o.profile ? tm.mark() : nullptr;
           launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
-                      o.max_local_iters, g->d_total + 3, st);
+                      g->d_mat, o.max_local_iters, g->d_total + 3, st);
           if (o.profile) push_ev.push_back({a, tm.mark()});
           advance();
           s.push_launches++;

@@ -37,7 +37,8 @@ struct Planes {
   const int *ct;     // input sink capacities
   unsigned char *lab;// canonical labels (1 = source side)
   long long stride;  // nodes per plane (all instances)
-  unsigned *rmask;   // bit d set iff cap[d] > 0 (kept exact by k_push)
+  unsigned *rmask;   // bit d: residual of arc d > 0; bit 8+d: plane d still
+                     // equals c0 (cap not yet written in this solve)
 };
 
 struct Tiles {
@@ -57,9 +58,11 @@ enum ShapeId {
 bool shape_is_3d(int shape);
 void shape_dims(int shape, int *tw, int *th, int *td);
 
-// Persistent work queues: ctr[2*slot] = count, ctr[2*slot+1] = head.  Every
-// consumer zeroes the other slot, so producer/consumer launches alternate
-// slots without any memset.
+// Persistent work queues.  Slot s of ctr (QSLOT ints) holds the list count at
+// [QSLOT*s] and the claim heads at [QSLOT*s + 8 ...]; every consumer zeroes
+// the other slot, so producer/consumer launches alternate slots without any
+// memset.
+constexpr int QSLOT = 16;
 struct Queue {
   int *list;
   int *ctr;
@@ -103,11 +106,11 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     hipStream_t st);
 void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
                        hipStream_t st);
-void launch_materialize(int shape, int nwg, const Tiles &T, const Planes &P,
-                        const Queue &q, int *fresh, hipStream_t st);
+// mat: per tile, 1 once every residual plane of the tile was written in
+// this solve (cleared when a solve starts).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
-                 const Queue &q, unsigned char *flag, int max_iters,
-                 long long *iter_total, hipStream_t st);
+                 const Queue &q, unsigned char *flag, unsigned char *mat,
+                 int max_iters, long long *iter_total, hipStream_t st);
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
                   hipStream_t st);

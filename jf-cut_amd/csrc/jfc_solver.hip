@@ -720,7 +720,7 @@ __global__ void k_compact(const int *src, int n, int reverse, unsigned char *fla
     f = flag[t] != 0;
     if (f) flag[t] = 0;
   }
-  wave_append(f, t, q.list, q.ctr + 2 * q.slot, sweep_cnt, total);
+  wave_append(f, t, q.list, q.ctr + QSLOT * q.slot, sweep_cnt, total);
 }
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
@@ -732,41 +732,56 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 
 // ---- persistent tile queue ----------------------------------------------------------
 
-// Workgroups reserve runs of `chunk` consecutive list entries with one
-// atomic, so a full pass over ~10^6 tiles costs ~10^4 same-address atomics
-// instead of 10^6, while late, short lists still spread one tile per group.
+// Workgroups claim runs of `chunk` consecutive list entries (16 for long
+// lists, down to 1 for short ones).  Each workgroup's first run is static
+// (group b takes run first(b) < G, G = grid size), so short or empty lists
+// cost no atomics.  Later runs are claimed dynamically, which balances tiles
+// of very different cost (a relabel visit iterates to its tile's fixpoint).
+// Workgroups are dispatched to the 8 XCDs round-robin (b % 8), so with 8
+// heads, head x serves the workgroups of XCD x: runs >= G form groups of 8
+// consecutive runs, group q belongs to head q % nheads, and a head hands out
+// its groups' runs in order.  Neighbouring tiles of a jump-flooding band
+// thus stay on one XCD's L2, and claims spread over 8 addresses.  Every run
+// belongs to one head and every head has workgroups, so the list is always
+// drained.  Slot s of `ctr` holds the list count at [QSLOT*s] and the heads
+// at [QSLOT*s + 8 ...]; the consumer of slot s zeroes the other slot for the
+// next producer.
 struct QueueCursor {
-  int count;
-  int chunk;
-  int *head;
-  __device__ __forceinline__ explicit QueueCursor(const Queue &q) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      q.ctr[2 * (q.slot ^ 1)] = 0;
-      q.ctr[2 * (q.slot ^ 1) + 1] = 0;
-    }
-    count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[2 * q.slot];
-    head = q.ctr + 2 * q.slot + 1;
+  int count, chunk, first, head_id, nheads;
+  int *heads;
+  int *s_run;  // LDS word broadcasting the claimed run
+  __device__ __forceinline__ QueueCursor(const Queue &q, int *lds_word) {
+    if (blockIdx.x == 0 && threadIdx.x < QSLOT) q.ctr[QSLOT * (q.slot ^ 1) + threadIdx.x] = 0;
+    count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
     chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
+    heads = q.ctr + QSLOT * q.slot + 8;
+    s_run = lds_word;
+    const int b = blockIdx.x;
+    nheads = (gridDim.x % 8 == 0) ? 8 : 1;
+    head_id = b % nheads;
+    if (gridDim.x % 64 == 0) {
+      const int w = b >> 3;
+      first = (((w >> 3) << 3) + (b & 7)) * 8 + (w & 7);
+    } else {
+      first = b;
+    }
   }
-  // Next list index for the whole workgroup (idx/end are uniform); false
-  // once the list is drained.  The first run of every workgroup is static
-  // (group b takes [b*chunk, (b+1)*chunk)), so groups of a short or empty
-  // list leave without touching the shared head; later runs are taken from
-  // the head, which starts past the static runs.  Between two calls the
-  // caller executes at least one barrier, so s_idx is never overwritten while
-  // being read.
-  __device__ __forceinline__ bool next(int &idx, int &end, int *s_idx) {
+  // Next list index for the whole workgroup (uniform); false once the list is
+  // drained.  Start with idx = -1, end = 0.  Between two calls the caller
+  // passes at least one barrier, so s_run is not overwritten while read.
+  __device__ __forceinline__ bool next(int &idx, int &end) {
     if (idx + 1 < end) {
       ++idx;
       return true;
     }
-    if (end == 0) {  // first call
-      idx = blockIdx.x * chunk;
-    } else {
-      if (threadIdx.x == 0) *s_idx = gridDim.x * chunk + atomicAdd(head, chunk);
+    int run = first;
+    if (idx >= 0) {
+      if (threadIdx.x == 0) *s_run = atomicAdd(&heads[head_id], 1);
       __syncthreads();
-      idx = *s_idx;
+      const int k = *s_run;  // k-th run handed out by this head
+      run = (int)gridDim.x + 8 * (head_id + nheads * (k >> 3)) + (k & 7);
     }
+    idx = run * chunk;
     end = min(idx + chunk, count);
     return idx < count;
   }
@@ -774,9 +789,14 @@ struct QueueCursor {
 
 // ---- residual-arc masks and lazy residual planes ----------------------------------
 //
-// rmask[v] bit d = (cap[d][v] > 0).  The relabel and cut searches only need
-// these bits, so they read 4 bytes per node instead of K planes.  mask0 is
-// the same for the input planes, built once per change of the inputs.
+// rmask[v] bit d = (residual of arc d of v > 0); bit 8 + d = "fresh": the
+// residual still equals the input c0[d][v] and cap[d][v] has not been
+// written in this solve.  The relabel and cut searches only need the low
+// bits, so they read 4 bytes per node instead of K planes.  mask0 holds the
+// bits of the input planes (every plane fresh), built once per change of the
+// inputs and copied to rmask when a solve starts; k_push reads fresh planes
+// from c0 and writes them to cap, so no separate reset copy of the residual
+// planes is needed.
 template <class S>
 __global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned *mask0) {
   const Lane<S> L(threadIdx.x);
@@ -788,7 +808,7 @@ __global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned 
     unsigned m = 0;
 #pragma unroll
     for (int d = 0; d < S::KMAX; ++d)
-      if (d < K && P.c0[d * P.stride + nd.v] > 0) m |= 1u << d;
+      if (d < K) m |= (P.c0[d * P.stride + nd.v] > 0 ? 1u : 0u) << d | 1u << (8 + d);
     mask0[nd.v] = m;
   }
 }
@@ -796,61 +816,6 @@ __global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned 
 void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
                        hipStream_t st) {
   JFC_DISPATCH(shape, (k_build_mask<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, mask0)));
-}
-
-// A solve starts with every tile "fresh": its residual planes equal the input
-// planes and live only in c0.  Before a push launch, each listed tile and
-// each neighbour it can flush into gets its planes copied c0 -> cap once
-// (one atomic exchange elects the copying group).  Tiles never pushed are
-// never copied.
-template <class S>
-__global__ __launch_bounds__(256) void k_materialize(Tiles T, Planes P, Queue q,
-                                                     int *fresh) {
-  __shared__ int s_take[S::D3 ? 7 : 9];
-  const Lane<S> L(threadIdx.x);
-  const int count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[2 * q.slot];
-  const long long st = P.stride;
-  for (int i = blockIdx.x; i < count; i += gridDim.x) {
-    const int t = q.list[i];
-    const TileRef r = tile_ref<S>(T, t);
-    const InstDesc &I = r.I;
-    constexpr int NC = S::D3 ? 7 : 9;
-    if (threadIdx.x < NC) {
-      int dx = 0, dy = 0, dz = 0;
-      if (S::D3) {
-        if (threadIdx.x > 0) dir_of<true>(threadIdx.x - 1, dx, dy, dz);
-      } else {
-        dx = (int)(threadIdx.x % 3) - 1;
-        dy = (int)(threadIdx.x / 3) - 1;
-      }
-      const int X = r.ltx + dx, Y = r.lty + dy, Z = r.ltz + dz;
-      int take = -1;
-      if (X >= 0 && X < I.tx && Y >= 0 && Y < I.ty && Z >= 0 && Z < I.tz) {
-        const int n = I.tile_off + X + I.tx * (Y + I.ty * Z);
-        if (atomicExch(&fresh[n], 0)) take = n;
-      }
-      s_take[threadIdx.x] = take;
-    }
-    __syncthreads();
-    for (int c = 0; c < NC; ++c) {
-      const int n = s_take[c];
-      if (n < 0) continue;
-      const TileRef rn = tile_ref<S>(T, n);
-      const int gx = rn.ox + L.lx, gy = rn.oy + L.ly, gz = rn.oz + L.lz;
-      if (gx < I.nx && gy < I.ny && gz < I.nz) {
-        const long long v = node_index(I, gx, gy, gz);
-#pragma unroll
-        for (int d = 0; d < S::KMAX; ++d)
-          if (d < I.K) P.cap[d * st + v] = P.c0[d * st + v];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-void launch_materialize(int shape, int nwg, const Tiles &T, const Planes &P,
-                        const Queue &q, int *fresh, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_materialize<S><<<nwg, 256, 0, st>>>(T, P, q, fresh)));
 }
 
 // ---- block-wise push-relabel -------------------------------------------------------
@@ -868,19 +833,18 @@ void launch_materialize(int shape, int nwg, const Tiles &T, const Planes &P,
 // max_iters iterations.
 template <class S>
 __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
-                                              unsigned char *flag, int max_iters,
-                                              long long *iter_total) {
+                                              unsigned char *flag, unsigned char *mat,
+                                              int max_iters, long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
-  __shared__ int s_idx;
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  const QueueCursor Q(q);
+  __shared__ int s_run;
+  QueueCursor Qc(q, &s_run);
   const long long st = P.stride;
-  QueueCursor Qc = Q;
-  int idx = 0, end = 0;
-  while (Qc.next(idx, end, &s_idx)) {
+  int idx = -1, end = 0;
+  while (Qc.next(idx, end)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
@@ -892,17 +856,32 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
     // (the role of cl_max_height in graphcut_push_relabel.cl, made exact).
     const long long nn = (long long)I.nx * I.ny * I.nz;
     const int hcap = nn < HINF ? (int)nn : HINF;
+    // A tile whose every plane is already written in this solve (mat[t])
+    // loads cap directly; otherwise the per-plane fresh bits pick c0 or cap.
+    const bool fresh_tile = mat[t] == 0;
     long long gv = 0;
     int e0 = 0, h0 = HINF;
+    unsigned rm = 0;
     int c0r[S::KMAX];
-    if (valid) {
-      gv = node_index(I, gx, gy, gz);
-      e0 = P.e[gv];
-      h0 = P.h[gv];
-    }
+    if (valid) gv = node_index(I, gx, gy, gz);
+    if (!fresh_tile) {
+      if (valid) {
+        e0 = P.e[gv];
+        h0 = P.h[gv];
+      }
 #pragma unroll
-    for (int d = 0; d < S::KMAX; ++d)
-      c0r[d] = (valid && d < K) ? P.cap[d * st + gv] : 0;
+      for (int d = 0; d < S::KMAX; ++d)
+        c0r[d] = (valid && d < K) ? P.cap[d * st + gv] : 0;
+    } else {
+      if (valid) {
+        e0 = P.e[gv];
+        h0 = P.h[gv];
+        rm = P.rmask[gv];
+      }
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        c0r[d] = (valid && d < K) ? ((rm >> (8 + d)) & 1u ? P.c0 : P.cap)[d * st + gv] : 0;
+    }
     s_e[L.me] = e0;
     s_h[L.me] = h0;
 #pragma unroll
@@ -980,16 +959,18 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
       const int e1 = s_e[L.me], h1 = s_h[L.me];
       if (e1 != e0) P.e[gv] = e1;
       if (h1 != h0) P.h[gv] = h1;
+      // fresh planes are written whether or not they changed; the mask word
+      // of a written tile holds residual bits only (exact, no fresh bits)
       unsigned m0 = 0, m1 = 0;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
         if (d < K) {
           const int c1 = s_c[d][L.me];
-          if (c1 != c0r[d]) P.cap[d * st + gv] = c1;
+          if (c1 != c0r[d] || ((rm >> (8 + d)) & 1u)) P.cap[d * st + gv] = c1;
           m0 |= (unsigned)(c0r[d] > 0) << d;
           m1 |= (unsigned)(c1 > 0) << d;
         }
-      if (m1 != m0) P.rmask[gv] = m1;
+      if (m1 != (fresh_tile ? rm : m0)) P.rmask[gv] = m1;
       act = e1 > 0 && h1 < HINF;
     }
     // flush flow that crossed into the halo: excess and the reverse residual
@@ -1007,28 +988,44 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
         atomicAdd(&P.e[g], f);
         flag[tile_of<S>(I, X, Y, Z)] = 1;
       }
-      unsigned bits = 0;
+      // Reverse residuals of the crossing arcs (> 0 now).  Plane d of halo
+      // node g is the arc from g into this tile, so no other tile of this
+      // colour class touches plane d of g or its two mask bits in this
+      // launch; other tiles may update other bits of the same mask word,
+      // hence the atomic xor of exactly the bits that change.
+      unsigned rg = 0, flip = 0;
+      bool loaded = false;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
         if (d < K) {
           const int dc = s_c[d][p];
           if (dc != 0) {
-            P.cap[d * st + g] += dc;
-            bits |= 1u << d;
+            if (!loaded) {
+              rg = P.rmask[g];
+              loaded = true;
+            }
+            const long long a = d * st + g;
+            if ((rg >> (8 + d)) & 1u) {
+              P.cap[a] = P.c0[a] + dc;
+              flip |= 1u << (8 + d);
+            } else {
+              P.cap[a] += dc;
+            }
+            if (!((rg >> d) & 1u)) flip |= 1u << d;
           }
         }
-      // the crossing arcs' reverse residuals are now > 0; several tiles of
-      // this colour class may flush into one halo node, hence the atomic
-      if (bits) atomicOr(&P.rmask[g], bits);
+      if (flip) atomicXor(&P.rmask[g], flip);
     }
     if (__syncthreads_or(act) && tid == 0) flag[t] = 1;
+    if (fresh_tile && tid == 0) mat[t] = 1;
   }
 }
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
-                 const Queue &q, unsigned char *flag, int max_iters,
-                 long long *iter_total, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, max_iters, iter_total)));
+                 const Queue &q, unsigned char *flag, unsigned char *mat,
+                 int max_iters, long long *iter_total, hipStream_t st) {
+  JFC_DISPATCH(shape,
+               (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, mat, max_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
@@ -1042,13 +1039,12 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty,
                                                unsigned char *active) {
   __shared__ int s_h[S::PADN];
-  __shared__ int s_idx;
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  const QueueCursor Q(q);
-  QueueCursor Qc = Q;
-  int idx = 0, end = 0;
-  while (Qc.next(idx, end, &s_idx)) {
+  __shared__ int s_run;
+  QueueCursor Qc(q, &s_run);
+  int idx = -1, end = 0;
+  while (Qc.next(idx, end)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
@@ -1148,14 +1144,13 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty, int *cert,
                                                const unsigned char *object) {
   __shared__ unsigned char s_l[S::PADN];
-  __shared__ int s_idx;
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  const QueueCursor Q(q);
-  QueueCursor Qc = Q;
+  __shared__ int s_run;
+  QueueCursor Qc(q, &s_run);
   constexpr unsigned kAxis = S::D3 ? 0x3fu : 0xfu;
-  int idx = 0, end = 0;
-  while (Qc.next(idx, end, &s_idx)) {
+  int idx = -1, end = 0;
+  while (Qc.next(idx, end)) {
     const int t = q.list[idx];
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
