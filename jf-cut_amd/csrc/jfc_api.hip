@@ -113,6 +113,9 @@ struct jfc_graph {
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
   long long *d_red = nullptr, *d_total = nullptr;
+  long long *d_def = nullptr;  // per instance: remaining deficit after the cut
+  int *d_row_off = nullptr;    // per instance: first row; [count] = rows
+  int nrows = 0;
   int *d_sweep = nullptr;
   int sweep_cap = 0;
   int last_sweeps = 0;
@@ -137,12 +140,15 @@ struct jfc_graph {
     release_tiles();
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
     dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
+    dfree(d_def); dfree(d_row_off);
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
   }
 
-  Tiles tiles() const { return Tiles{d_inst, d_tile_inst, ntiles}; }
+  Tiles tiles() const {
+    return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows};
+  }
   Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot, rmask}; }
   long long inst_nodes(int i) const {
     return (long long)inst[i].nx * inst[i].ny * inst[i].nz;
@@ -315,9 +321,27 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->rmask, N, "residual masks")) ||
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
       (rc = dalloc(&g->d_total, 5, "counters")) ||
+      (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
+      (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
     delete g;
     return rc;
+  }
+  {
+    std::vector<int> row_off(count + 1, 0);
+    long long rows = 0;
+    for (int i = 0; i < count; ++i) {
+      row_off[i] = (int)rows;
+      rows += (long long)g->inst[i].ny * g->inst[i].nz;
+    }
+    if (rows >= (1LL << 31)) { delete g; return fail(JFC_ERR_INVALID, "graph too large"); }
+    row_off[count] = (int)rows;
+    g->nrows = (int)rows;
+    if (hipMemcpy(g->d_row_off, row_off.data(), row_off.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      delete g;
+      return fail(JFC_ERR_DEVICE, "row table upload failed");
+    }
   }
   launch_fill_i32(g->cs, off, 0, 0);
   launch_fill_i32(g->ct, off, 0, 0);
@@ -618,7 +642,7 @@ struct Solver {
   int canonical_cut(int *found) {
     hipEvent_t a = tm.mark();
     launch_fill_i32(g->d_flagword, 1, 0, st);
-    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
+    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, st);
     long long tiles = 0;
     int rc = search(kCut, &s.cut_passes, &tiles);
     if (rc) return rc;
@@ -735,16 +759,19 @@ struct Solver {
       int found = 0;
       if ((rc = canonical_cut(&found))) return rc;
     }
-    launch_deficit(g->shape, T, P, g->d_red, st);
+    // the last cut search ran on the final excess and left the remaining
+    // deficit of every instance in d_def
     hipEvent_t t1 = tm.mark();
+    std::vector<long long> def((size_t)nI);
     JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
+                           hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipMemcpyAsync(def.data(), g->d_def, def.size() * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
     long long totals[4] = {0, 0, 0, 0};
     JFC_HIP(hipMemcpyAsync(totals, g->d_total, sizeof(totals), hipMemcpyDeviceToHost, st));
     JFC_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < nI; ++i)
-      g->flows[i] = red[i * RED_SLOTS + RED_CONST] + red[i * RED_SLOTS + RED_DEF0] -
-                    red[i * RED_SLOTS + RED_DEF];
+      g->flows[i] = red[i * RED_SLOTS + RED_CONST] + red[i * RED_SLOTS + RED_DEF0] - def[i];
     s.flow = g->flows[0];
     s.converged = done ? 1 : 0;
     s.push_tiles = totals[0];
@@ -914,7 +941,10 @@ extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
   launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr, 0);
   unsigned char *src = g->d_flag;
   if (which == 0) { launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, 0); src = g->d_seed; }
-  if (which == 1) { launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, 0); src = g->d_seed; }
+  if (which == 1) {
+    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, 0);
+    src = g->d_seed;
+  }
   JFC_HIP(hipDeviceSynchronize());
   JFC_HIP(hipMemcpy(flags, src, g->ntiles, hipMemcpyDeviceToHost));
   g->solved = false;

@@ -12,9 +12,9 @@ namespace jfc {
 constexpr int HINF = 0x3fffffff;
 
 // Per-instance reduction slots (int64): folded constant, initial deficit,
-// final deficit, energy, initial supply.
-constexpr int RED_CONST = 0, RED_DEF0 = 1, RED_DEF = 2, RED_ENERGY = 3,
-              RED_SUP0 = 4, RED_SLOTS = 5;
+// energy, initial supply.  (The final deficit has an array of its own.)
+constexpr int RED_CONST = 0, RED_DEF0 = 1, RED_ENERGY = 3, RED_SUP0 = 4,
+              RED_SLOTS = 5;
 
 // One grid instance inside a graph.  All instances share the plane arrays;
 // instance i owns nodes [node_off, node_off + nx*ny*nz) and tiles
@@ -45,6 +45,9 @@ struct Tiles {
   const InstDesc *inst;
   const int *tile_inst;  // instance of every tile
   int ntiles;
+  const int *row_off;    // first grid row (y, z) of every instance, + total
+  int ninst;
+  int nrows;             // rows of nx nodes over all instances
 };
 
 // Tile shapes (256 nodes each, one node per lane).
@@ -90,10 +93,10 @@ void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
 
 void launch_gr_init(int shape, const Tiles &T, const Planes &P,
                     unsigned char *seed, hipStream_t st);
+// Also sums the remaining deficit max(-e, 0) per instance into def[inst]
+// (def must be zeroed by the caller).
 void launch_cut_init(int shape, const Tiles &T, const Planes &P,
-                     unsigned char *seed, hipStream_t st);
-void launch_count_active(int shape, const Tiles &T, const Planes &P,
-                         unsigned char *flag, hipStream_t st);
+                     unsigned char *seed, long long *def, hipStream_t st);
 
 // Jump-flooding tile distance + counting sort of tiles by distance.
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
@@ -124,8 +127,6 @@ void launch_tag_init(int shape, const Tiles &T, unsigned char *tag,
 void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
                 const Queue &q, unsigned char *dirty, const unsigned char *object,
                 hipStream_t st);
-void launch_deficit(int shape, const Tiles &T, const Planes &P, long long *red,
-                    hipStream_t st);
 void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
                    hipStream_t st);
 void launch_backward(int shape, const Tiles &T, const Planes &P,

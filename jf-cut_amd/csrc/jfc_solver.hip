@@ -6,17 +6,19 @@
 //   k_gr_init  graphcut_init_global_relabel  graphcut_global_relabel.cl:19-48
 //   k_reach    graphcut_compute_cut    cl/GraphCut/graphcut_compute_cut.cl:51-140, :182-282
 //   k_cut_init graphcut_init_cut       graphcut_compute_cut.cl:19-49
-//   k_count_active / k_compact  graphcut_count_node + clogs scan + graphcut_scatter_node
+//   k_compact  graphcut_count_node + clogs scan + graphcut_scatter_node
 //                               cl/GraphCut/graphcut_count_block.cl:19-65
 //   k_jf_*     graphcut_bfs_jump_flooding / _histogram / _mapping
 //              cl/GraphCut/graphcut_bfs_block.cl:43-91 (+ reduce_histogram, INF reduce.cl:56-141)
-//   k_deficit  reduce_flow  bin/x64/cl/Infrastructures/reduce.cl:17-42 (int64 here)
+//   k_cut_init also reduce_flow  bin/x64/cl/Infrastructures/reduce.cl:17-42 (int64 here)
 //
 // Layout: every graph stores int32 planes (e, h, K residual planes) in
 // structure-of-arrays order over all instances.  Kernels work on tiles of
-// 256 nodes (one node per lane) staged in LDS with a one-node halo; tiles of
-// one colour class never share a node or halo cell, so a push-relabel launch
-// over one colour class needs no global atomics except for halo excess.
+// 256 nodes (one node per lane) staged in LDS with a one-node halo.  Tiles of
+// one colour class never share a node; a halo node shared by two of them is
+// reached through different arcs, so within a push-relabel launch every
+// residual plane entry has one writer and only excess and mask words need
+// atomics.
 #include "jfc_device.h"
 
 #include <algorithm>
@@ -194,114 +196,322 @@ __device__ __forceinline__ TileNode tile_node(const Tiles &T, int t, int t_end,
   return n;
 }
 
-__device__ __forceinline__ void fold_flush(long long *red, int inst, long long a,
-                                           long long b, long long c, long long *buf) {
-  const long long A = block_sum_ll(a, buf);
-  const long long B = block_sum_ll(b, buf);
-  const long long C = block_sum_ll(c, buf);
-  if (threadIdx.x == 0) {
-    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_CONST], (unsigned long long)A);
-    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_DEF0], (unsigned long long)B);
-    atomicAdd((unsigned long long *)&red[inst * RED_SLOTS + RED_SUP0], (unsigned long long)C);
+// ---- row-streaming passes ----------------------------------------------------
+//
+// Whole-graph passes (fold, relabel / cut initialisation, deficit, masks) walk
+// grid rows, not tiles: row r of the graph (the rows of all instances
+// concatenated) is nx consecutive nodes, so a workgroup streams it with
+// 16-byte loads and stores, several in flight per lane.  A lane handles a
+// group of up to 4 consecutive nodes: an unaligned head group, aligned
+// groups of 4, a tail group.  Per-tile flags are derived from the column and
+// the row's tile row; per-instance sums are flushed when a workgroup's row
+// changes instance and at its end.
+
+// Rows per workgroup step and groups in flight per lane.
+constexpr int RS_U = 4;
+
+template <class S>
+struct RowRef {
+  int inst, nx;
+  long long v0;  // first node of the row
+  int tile0;     // tile of column 0
+};
+
+template <class S>
+__device__ __forceinline__ RowRef<S> row_ref(const Tiles &T, int r) {
+  int lo = 0, hi = T.ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.row_off[mid] <= r) lo = mid;
+    else hi = mid - 1;
   }
+  const InstDesc &I = T.inst[lo];
+  const int rr = r - T.row_off[lo];
+  const int y = rr % I.ny, z = rr / I.ny;
+  RowRef<S> R;
+  R.inst = lo;
+  R.nx = I.nx;
+  R.v0 = I.node_off + (long long)rr * I.nx;
+  R.tile0 = I.tile_off + I.tx * (y / S::TH + I.ty * (z / S::TD));
+  return R;
 }
+
+// Group k of a row: first column x and node count n (0 past the end).
+struct Group {
+  int x, n;
+};
+
+__device__ __forceinline__ Group row_group(long long v0, int nx, int k) {
+  const int head = min((int)((4 - (v0 & 3)) & 3), nx);
+  Group g;
+  if (head > 0) {
+    if (k == 0) { g.x = 0; g.n = head; return g; }
+    g.x = head + 4 * (k - 1);
+  } else {
+    g.x = 4 * k;
+  }
+  g.n = max(0, min(4, nx - g.x));
+  return g;
+}
+
+__device__ __forceinline__ int row_groups(long long v0, int nx) {
+  const int head = min((int)((4 - (v0 & 3)) & 3), nx);
+  return (head > 0 ? 1 : 0) + (nx - head + 3) / 4;
+}
+
+// Load / store up to 4 consecutive ints (16-byte access when the group is
+// full and aligned).
+__device__ __forceinline__ int4 load4(const int *p, long long v, int n) {
+  int4 r = make_int4(0, 0, 0, 0);
+  if (n == 4 && (((size_t)(p + v)) & 15) == 0) return *(const int4 *)(p + v);
+  if (n > 0) r.x = p[v];
+  if (n > 1) r.y = p[v + 1];
+  if (n > 2) r.z = p[v + 2];
+  if (n > 3) r.w = p[v + 3];
+  return r;
+}
+
+__device__ __forceinline__ void store4(int *p, long long v, int n, int4 a) {
+  if (n == 4 && (((size_t)(p + v)) & 15) == 0) { *(int4 *)(p + v) = a; return; }
+  if (n > 0) p[v] = a.x;
+  if (n > 1) p[v + 1] = a.y;
+  if (n > 2) p[v + 2] = a.z;
+  if (n > 3) p[v + 3] = a.w;
+}
+
+__device__ __forceinline__ void store4_u8(unsigned char *p, long long v, int n, unsigned bits) {
+  if (n == 4 && (((size_t)(p + v)) & 3) == 0) {
+    *(unsigned *)(p + v) = (bits & 1u) | ((bits >> 1) & 1u) << 8 | ((bits >> 2) & 1u) << 16 |
+                           ((bits >> 3) & 1u) << 24;
+    return;
+  }
+  for (int i = 0; i < n; ++i) p[v + i] = (unsigned char)((bits >> i) & 1u);
+}
+
+__device__ __forceinline__ int comp(const int4 &a, int i) {
+  return i == 0 ? a.x : i == 1 ? a.y : i == 2 ? a.z : a.w;
+}
+
+// Set flag[tile] = 1 for the tiles of the nodes of a group with a set bit.
+template <class S>
+__device__ __forceinline__ void flag_group(unsigned char *flag, int tile0, Group g,
+                                           unsigned bits) {
+  if (!bits) return;
+  const int t0 = tile0 + g.x / S::TW, t1 = tile0 + (g.x + g.n - 1) / S::TW;
+  if (t0 == t1) {
+    flag[t0] = 1;
+    return;
+  }
+  for (int i = 0; i < g.n; ++i)
+    if ((bits >> i) & 1u) flag[tile0 + (g.x + i) / S::TW] = 1;
+}
+
+// Per-instance int64 sums of a workgroup: accumulated per lane, flushed with
+// one atomic per workgroup and instance.
+template <int NS>
+struct InstSums {
+  long long a[NS];
+  int cur = -1;
+  __device__ __forceinline__ void reset() {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) a[k] = 0;
+  }
+  __device__ __forceinline__ void flush(long long *const *dst, const int *stride,
+                                        long long *buf) {
+    if (cur < 0) return;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const long long A = block_sum_ll(a[k], buf);
+      if (threadIdx.x == 0 && A != 0)
+        atomicAdd((unsigned long long *)&dst[k][(long long)cur * stride[k]], (unsigned long long)A);
+    }
+  }
+  // Called with the (uniform) instance of the next row.
+  __device__ __forceinline__ void enter(int inst, long long *const *dst, const int *stride,
+                                        long long *buf) {
+    if (inst == cur) return;
+    flush(dst, stride, buf);
+    cur = inst;
+    reset();
+  }
+};
+
+static int row_grid(int nrows) { return nrows < 2048 ? (nrows > 0 ? nrows : 1) : 2048; }
 
 // Fold terminal arcs: e = cs - ct; const += min(cs, ct);
 // deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
 // With `active` set it also starts the labelling at zero heights (the
-// reference clears h, QGCWidget.cpp:2400) and flags the tiles holding excess.
+// reference clears h, QGCWidget.cpp:2400) and flags the tiles holding excess
+// (`active` zeroed by the launcher).
 template <class S>
 __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
                                               unsigned char *active) {
   __shared__ long long buf[4];
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  __shared__ int s_w[8];
-  int par = 0;
-  int cur = -1;
-  long long a = 0, b = 0, c = 0;
-  for (int t = C.t0; t < C.t1; t += TU) {
-    TileNode nd[TU];
-    int s[TU], k[TU];
+  long long *dst[3] = {red + RED_CONST, red + RED_DEF0, red + RED_SUP0};
+  const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
+  InstSums<3> sums;
+  sums.reset();
+  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
+    const RowRef<S> R = row_ref<S>(T, r);
+    sums.enter(R.inst, dst, stride, buf);
+    const int ng = row_groups(R.v0, R.nx);
+    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
+      Group g[RS_U];
+      int4 a[RS_U], b[RS_U];
 #pragma unroll
-    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
-#pragma unroll
-    for (int u = 0; u < TU; ++u) {
-      s[u] = nd[u].valid ? P.cs[nd[u].v] : 0;
-      k[u] = nd[u].valid ? P.ct[nd[u].v] : 0;
-    }
-    int bits = 0;
-#pragma unroll
-    for (int u = 0; u < TU; ++u) {
-      if (nd[u].inst < 0) break;
-      if (nd[u].inst != cur) {
-        if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
-        cur = nd[u].inst;
-        a = b = c = 0;
+      for (int u = 0; u < RS_U; ++u) {
+        const int k = k0 + u * 256;
+        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
+        a[u] = load4(P.cs, R.v0 + g[u].x, g[u].n);
+        b[u] = load4(P.ct, R.v0 + g[u].x, g[u].n);
       }
-      if (nd[u].valid) {
-        P.e[nd[u].v] = s[u] - k[u];
-        if (active) P.h[nd[u].v] = 0;
-        bits |= (s[u] > k[u]) << u;
-        a += min(s[u], k[u]);
-        b += max(k[u] - s[u], 0);
-        c += max(s[u] - k[u], 0);
+#pragma unroll
+      for (int u = 0; u < RS_U; ++u) {
+        if (g[u].n == 0) continue;
+        const long long v = R.v0 + g[u].x;
+        int4 e;
+        e.x = a[u].x - b[u].x; e.y = a[u].y - b[u].y;
+        e.z = a[u].z - b[u].z; e.w = a[u].w - b[u].w;
+        store4(P.e, v, g[u].n, e);
+        if (active) store4(P.h, v, g[u].n, make_int4(0, 0, 0, 0));
+        unsigned bits = 0;
+        for (int i = 0; i < g[u].n; ++i) {
+          const int s = comp(a[u], i), t = comp(b[u], i);
+          sums.a[0] += min(s, t);
+          sums.a[1] += max(t - s, 0);
+          sums.a[2] += max(s - t, 0);
+          bits |= (unsigned)(s > t) << i;
+        }
+        if (active) flag_group<S>(active, R.tile0, g[u], bits);
       }
-    }
-    if (active) {
-      bits = block_or_bits(bits, s_w, par ^= 1);
-      if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
-        active[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
     }
   }
-  if (cur >= 0) fold_flush(red, cur, a, b, c, buf);
+  sums.flush(dst, stride, buf);
 }
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
                  unsigned char *active, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_fold<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red, active)));
+  if (active) launch_fill_u8(active, T.ntiles, 0, st);
+  JFC_DISPATCH(shape, (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active)));
 }
 
-// Remaining deficit per instance: sum of max(-e, 0).
+// graphcut_init_cut (graphcut_compute_cut.cl:19-49): label = 1 where
+// excess > 0, tiles holding such a node seed the jump flooding (`seed`
+// zeroed by the launcher).  The same pass sums the remaining deficit
+// max(-e, 0) per instance into def[inst] (reduce_flow, reduce.cl:17-42, in
+// int64): the cut search is the last step of a solve and reads the final
+// excess.
 template <class S>
-__global__ __launch_bounds__(256) void k_deficit(Tiles T, Planes P, long long *red) {
+__global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned char *seed,
+                                                  long long *def) {
   __shared__ long long buf[4];
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  int cur = -1;
-  long long a = 0;
-  for (int t = C.t0; t < C.t1; t += TU) {
-    TileNode nd[TU];
-    int e[TU];
+  long long *dst[1] = {def};
+  const int stride[1] = {1};
+  InstSums<1> sums;
+  sums.reset();
+  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
+    const RowRef<S> R = row_ref<S>(T, r);
+    sums.enter(R.inst, dst, stride, buf);
+    const int ng = row_groups(R.v0, R.nx);
+    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
+      Group g[RS_U];
+      int4 e[RS_U];
 #pragma unroll
-    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
-#pragma unroll
-    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
-#pragma unroll
-    for (int u = 0; u < TU; ++u) {
-      if (nd[u].inst < 0) break;
-      if (nd[u].inst != cur) {
-        if (cur >= 0) {
-          const long long A = block_sum_ll(a, buf);
-          if (threadIdx.x == 0)
-            atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
-        }
-        cur = nd[u].inst;
-        a = 0;
+      for (int u = 0; u < RS_U; ++u) {
+        const int k = k0 + u * 256;
+        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
+        e[u] = load4(P.e, R.v0 + g[u].x, g[u].n);
       }
-      a += max(-e[u], 0);
+#pragma unroll
+      for (int u = 0; u < RS_U; ++u) {
+        if (g[u].n == 0) continue;
+        unsigned bits = 0;
+        for (int i = 0; i < g[u].n; ++i) {
+          const int x = comp(e[u], i);
+          sums.a[0] += max(-x, 0);
+          bits |= (unsigned)(x > 0) << i;
+        }
+        store4_u8(P.lab, R.v0 + g[u].x, g[u].n, bits);
+        flag_group<S>(seed, R.tile0, g[u], bits);
+      }
     }
   }
-  if (cur >= 0) {
-    const long long A = block_sum_ll(a, buf);
-    if (threadIdx.x == 0)
-      atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_DEF], (unsigned long long)A);
+  sums.flush(dst, stride, buf);
+}
+
+void launch_cut_init(int shape, const Tiles &T, const Planes &P,
+                     unsigned char *seed, long long *def, hipStream_t st) {
+  launch_fill_u8(seed, T.ntiles, 0, st);
+  launch_fill_i32((int *)def, 2LL * T.ninst, 0, st);
+  JFC_DISPATCH(shape,
+               (k_cut_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, seed, def)));
+}
+
+// graphcut_init_global_relabel (graphcut_global_relabel.cl:19-48): h = 0 at
+// deficit nodes, unvisited elsewhere; tiles holding a deficit node seed the
+// jump flooding (`seed` zeroed by the launcher).
+template <class S>
+__global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
+  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
+    const RowRef<S> R = row_ref<S>(T, r);
+    const int ng = row_groups(R.v0, R.nx);
+    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
+      Group g[RS_U];
+      int4 e[RS_U];
+#pragma unroll
+      for (int u = 0; u < RS_U; ++u) {
+        const int k = k0 + u * 256;
+        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
+        e[u] = load4(P.e, R.v0 + g[u].x, g[u].n);
+      }
+#pragma unroll
+      for (int u = 0; u < RS_U; ++u) {
+        if (g[u].n == 0) continue;
+        int4 h;
+        h.x = e[u].x < 0 ? 0 : HINF; h.y = e[u].y < 0 ? 0 : HINF;
+        h.z = e[u].z < 0 ? 0 : HINF; h.w = e[u].w < 0 ? 0 : HINF;
+        store4(P.h, R.v0 + g[u].x, g[u].n, h);
+        unsigned bits = 0;
+        for (int i = 0; i < g[u].n; ++i) bits |= (unsigned)(comp(e[u], i) < 0) << i;
+        flag_group<S>(seed, R.tile0, g[u], bits);
+      }
+    }
   }
 }
 
-void launch_deficit(int shape, const Tiles &T, const Planes &P, long long *red,
-                    hipStream_t st) {
-  JFC_DISPATCH(shape, (k_deficit<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+void launch_gr_init(int shape, const Tiles &T, const Planes &P,
+                    unsigned char *seed, hipStream_t st) {
+  launch_fill_u8(seed, T.ntiles, 0, st);
+  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, seed)));
+}
+
+// Residual-arc bits of the input planes (see the mask comment below).
+template <class S>
+__global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned *mask0) {
+  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
+    const RowRef<S> R = row_ref<S>(T, r);
+    const int K = T.inst[R.inst].K;
+    const int ng = row_groups(R.v0, R.nx);
+    for (int k = threadIdx.x; k < ng; k += 256) {
+      const Group g = row_group(R.v0, R.nx, k);
+      const long long v = R.v0 + g.x;
+      int4 m = make_int4(0, 0, 0, 0);
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) {
+        if (d >= K) break;
+        const int4 c = load4(P.c0 + d * P.stride, v, g.n);
+        const int fresh = 1 << (8 + d);
+        m.x |= (c.x > 0 ? 1 << d : 0) | fresh; m.y |= (c.y > 0 ? 1 << d : 0) | fresh;
+        m.z |= (c.z > 0 ? 1 << d : 0) | fresh; m.w |= (c.w > 0 ? 1 << d : 0) | fresh;
+      }
+      store4((int *)mask0, v, g.n, m);
+    }
+  }
+}
+
+void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
+                       hipStream_t st) {
+  JFC_DISPATCH(shape, (k_build_mask<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, mask0)));
 }
 
 // Cut energy on the input capacities (QGCWidget.cpp:2452-2520).
@@ -408,72 +618,6 @@ void launch_backward_all(int shape, const Tiles &T, const Planes &P, float *dcs,
 
 // ---- initialisation of relabel / cut searches --------------------------------
 
-// graphcut_init_global_relabel: h = 0 at deficit nodes, unvisited elsewhere;
-// tiles holding a deficit node seed the jump flooding.
-template <class S>
-__global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  __shared__ int s_w[8];
-  int par = 0;
-  for (int t = C.t0; t < C.t1; t += TU) {
-    TileNode nd[TU];
-    int e[TU];
-#pragma unroll
-    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
-#pragma unroll
-    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
-    int bits = 0;
-#pragma unroll
-    for (int u = 0; u < TU; ++u)
-      if (nd[u].valid) {
-        P.h[nd[u].v] = e[u] < 0 ? 0 : HINF;
-        bits |= (e[u] < 0) << u;
-      }
-    bits = block_or_bits(bits, s_w, par ^= 1);
-    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
-      seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
-  }
-}
-
-void launch_gr_init(int shape, const Tiles &T, const Planes &P,
-                    unsigned char *seed, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_gr_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, seed)));
-}
-
-// graphcut_init_cut: label = 1 where excess > 0; tiles holding one seed the
-// jump flooding.
-template <class S>
-__global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned char *seed) {
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  __shared__ int s_w[8];
-  int par = 0;
-  for (int t = C.t0; t < C.t1; t += TU) {
-    TileNode nd[TU];
-    int e[TU];
-#pragma unroll
-    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
-#pragma unroll
-    for (int u = 0; u < TU; ++u) e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
-    int bits = 0;
-#pragma unroll
-    for (int u = 0; u < TU; ++u)
-      if (nd[u].valid) {
-        P.lab[nd[u].v] = (unsigned char)(e[u] > 0);
-        bits |= (e[u] > 0) << u;
-      }
-    bits = block_or_bits(bits, s_w, par ^= 1);
-    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
-      seed[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
-  }
-}
-
-void launch_cut_init(int shape, const Tiles &T, const Planes &P,
-                     unsigned char *seed, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_cut_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, seed)));
-}
-
 // graphcut_init_tag (graphcut_compute_tag.cl:19-49): the tag plane arrives
 // holding the foreground seed bytes; it leaves as 0/1, and tiles with a seed
 // seed the jump flooding.
@@ -507,39 +651,6 @@ __global__ __launch_bounds__(256) void k_tag_init(Tiles T, unsigned char *tag,
 void launch_tag_init(int shape, const Tiles &T, unsigned char *tag, unsigned char *seed,
                      hipStream_t st) {
   JFC_DISPATCH(shape, (k_tag_init<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, tag, seed)));
-}
-
-// graphcut_count_node: a tile is active if one of its nodes has excess and a
-// finite height.
-template <class S>
-__global__ __launch_bounds__(256) void k_count_active(Tiles T, Planes P,
-                                                      unsigned char *flag) {
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  __shared__ int s_w[8];
-  int par = 0;
-  for (int t = C.t0; t < C.t1; t += TU) {
-    TileNode nd[TU];
-    int e[TU], h[TU];
-#pragma unroll
-    for (int u = 0; u < TU; ++u) nd[u] = tile_node<S>(T, t + u, C.t1, L);
-#pragma unroll
-    for (int u = 0; u < TU; ++u) {
-      e[u] = nd[u].valid ? P.e[nd[u].v] : 0;
-      h[u] = nd[u].valid ? P.h[nd[u].v] : HINF;
-    }
-    int bits = 0;
-#pragma unroll
-    for (int u = 0; u < TU; ++u) bits |= (e[u] > 0 && h[u] < HINF) << u;
-    bits = block_or_bits(bits, s_w, par ^= 1);
-    if (threadIdx.x < TU && t + (int)threadIdx.x < C.t1)
-      flag[t + threadIdx.x] = (unsigned char)((bits >> threadIdx.x) & 1);
-  }
-}
-
-void launch_count_active(int shape, const Tiles &T, const Planes &P,
-                         unsigned char *flag, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_count_active<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, flag)));
 }
 
 // ---- jump flooding over the tile grid + counting sort --------------------------
@@ -797,27 +908,6 @@ struct QueueCursor {
 // inputs and copied to rmask when a solve starts; k_push reads fresh planes
 // from c0 and writes them to cap, so no separate reset copy of the residual
 // planes is needed.
-template <class S>
-__global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned *mask0) {
-  const Lane<S> L(threadIdx.x);
-  const Chunk C(T.ntiles);
-  for (int t = C.t0; t < C.t1; ++t) {
-    const TileNode nd = tile_node<S>(T, t, C.t1, L);
-    if (!nd.valid) continue;
-    const int K = T.inst[nd.inst].K;
-    unsigned m = 0;
-#pragma unroll
-    for (int d = 0; d < S::KMAX; ++d)
-      if (d < K) m |= (P.c0[d * P.stride + nd.v] > 0 ? 1u : 0u) << d | 1u << (8 + d);
-    mask0[nd.v] = m;
-  }
-}
-
-void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
-                       hipStream_t st) {
-  JFC_DISPATCH(shape, (k_build_mask<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, mask0)));
-}
-
 // ---- block-wise push-relabel -------------------------------------------------------
 //
 // One workgroup discharges one tile at a time: e, h and the K residual planes
