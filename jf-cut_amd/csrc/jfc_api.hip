@@ -82,6 +82,12 @@ int shape_from_option(int opt, int dim) {
   }
 }
 
+// Device counters (int64): [0] push tile visits, [1] relabel tile visits,
+// [2] cut tile visits, [4] tag tile visits, [kIterSlots, +64) local
+// iterations of the push kernel (workgroup b adds to slot b % 64, so no
+// single address takes one atomic per workgroup).
+constexpr int kIterSlots = 5, kTotals = kIterSlots + 64;
+
 // algorithmic HBM bytes per node of one push-relabel visit: read and write
 // e, h and K residual planes (int32)
 long long push_bytes_per_node(int K) { return 2LL * 4 * (2 + K); }
@@ -259,17 +265,14 @@ extern "C" void jfc_default_options(jfc_options *o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->max_sweeps = 1 << 20;
-  o->check_every = 4;
-  o->relabel_work = 4.0;
-  o->max_local_iters = 64;
+  o->check_every = 1;
+  o->relabel_work = 2.0;
+  o->max_local_iters = 2;
   o->tile_shape = JFC_TILE_DEFAULT;
   o->profile = 0;
   o->stream = nullptr;
   o->initial_relabel = 0;
   o->stall_ratio = 0.99;
-  o->check_every = 1;
-  o->max_local_iters = 4;
-  o->relabel_work = 2.0;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -320,7 +323,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->mask0, N, "residual masks")) ||
       (rc = dalloc(&g->rmask, N, "residual masks")) ||
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
-      (rc = dalloc(&g->d_total, 5, "counters")) ||
+      (rc = dalloc(&g->d_total, kTotals, "counters")) ||
       (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
       (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
@@ -665,7 +668,7 @@ struct Solver {
     hipEvent_t t0 = tm.mark();
     launch_fill_i32(g->d_ctr, 2 * QSLOT, 0, st);
     launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
-    launch_fill_i32((int *)g->d_total, 10, 0, st);
+    launch_fill_i32((int *)g->d_total, 2 * kTotals, 0, st);
     launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
     g->slot = 0;
     // the residual-arc masks restart from the inputs' with every plane
@@ -708,7 +711,7 @@ struct Solver {
                          g->d_sweep + sweep, g->d_total, st);
           hipEvent_t a = o.profile ? tm.mark() : nullptr;
           launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
-                      g->d_mat, o.max_local_iters, g->d_total + 3, st);
+                      g->d_mat, o.max_local_iters, g->d_total + kIterSlots, st);
           if (o.profile) push_ev.push_back({a, tm.mark()});
           advance();
           s.push_launches++;
@@ -767,7 +770,7 @@ struct Solver {
                            hipMemcpyDeviceToHost, st));
     JFC_HIP(hipMemcpyAsync(def.data(), g->d_def, def.size() * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
-    long long totals[4] = {0, 0, 0, 0};
+    long long totals[kTotals];
     JFC_HIP(hipMemcpyAsync(totals, g->d_total, sizeof(totals), hipMemcpyDeviceToHost, st));
     JFC_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < nI; ++i)
@@ -775,7 +778,8 @@ struct Solver {
     s.flow = g->flows[0];
     s.converged = done ? 1 : 0;
     s.push_tiles = totals[0];
-    s.local_iters = totals[3];
+    s.local_iters = 0;
+    for (int k = 0; k < 64; ++k) s.local_iters += totals[kIterSlots + k];
     g->last_sweeps = sweep;
     s.relabel_tiles += 0;
     s.push_bytes = totals[0] * 256LL * push_bytes_per_node(g->Kg);

@@ -16,6 +16,7 @@ struct Shape {
   static constexpr int PADN = PW * PH * PD;
   static constexpr int NT = TW * TH * TD;
   static constexpr int KMAX = D3 ? 6 : 8;
+  static constexpr int NCELL = (PADN + NT - 1) / NT;  // padded cells per lane
   static_assert(NT == 256, "one node per lane of a 256-lane workgroup");
 };
 
@@ -67,7 +68,8 @@ struct TileRef {
 template <class S>
 __device__ __forceinline__ TileRef tile_ref(const Tiles &T, int t) {
   TileRef r;
-  r.I = T.inst[T.tile_inst[t]];
+  // one instance: the descriptor does not depend on t (no dependent load)
+  r.I = T.ninst == 1 ? T.inst[0] : T.inst[T.tile_inst[t]];
   const int lt = t - r.I.tile_off;
   r.ltx = lt % r.I.tx;
   const int q = lt / r.I.tx;
@@ -122,6 +124,33 @@ template <class S>
 __device__ __forceinline__ bool halo_needed(int nout, int K) {
   if (nout == 1) return true;
   return !S::D3 && nout == 2 && K == 8;
+}
+
+// Padded cell p of a tile as seen by one lane: halo (not interior), and
+// live = a halo cell the connectivity needs that lies inside the instance,
+// with its node index (0 otherwise, a valid dummy address for loads).
+struct HaloCell {
+  bool halo, live;
+  long long g;
+};
+
+template <class S>
+__device__ __forceinline__ HaloCell halo_cell(const InstDesc &I, int ox, int oy, int oz,
+                                              int p, int K) {
+  HaloCell c{false, false, 0};
+  if (p >= S::PADN) return c;
+  int x, y, z, nout;
+  bool inter;
+  pad_coords<S>(p, x, y, z, inter, nout);
+  if (inter) return c;
+  c.halo = true;
+  if (!halo_needed<S>(nout, K)) return c;
+  const int X = ox + x, Y = oy + y, Z = oz + z;
+  if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz) {
+    c.live = true;
+    c.g = node_index(I, X, Y, Z);
+  }
+  return c;
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -110,7 +110,8 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
                        hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in
-// this solve (cleared when a solve starts).
+// this solve (cleared when a solve starts).  iter_total: 64 int64 counters
+// of local iterations (or null).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
                  int max_iters, long long *iter_total, hipStream_t st);

@@ -819,10 +819,20 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
   k_scatter_order<<<gh, 256, 2 * nbins * sizeof(int), st>>>(tdist, n, nbins, hist, order);
 }
 
-// ---- list compaction (ballot + one atomic per wave) ------------------------------
+// ---- list compaction (ballots + one atomic per 1024-lane block) ------------------
+//
+// Flagged entries of `src` (in order, or reversed) are appended to the queue
+// list and their flags cleared.  Each wave counts its hits with a ballot; the
+// block adds its total to the list count (and to the sweep and run totals)
+// with one atomic each, so a pass over 10^5..10^6 tiles costs a few hundred
+// same-address atomics.
+constexpr int kCompactBlock = 1024;
 
-__global__ void k_compact(const int *src, int n, int reverse, unsigned char *flag,
-                          Queue q, int *sweep_cnt, long long *total) {
+__global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n, int reverse,
+                                                           unsigned char *flag, Queue q,
+                                                           int *sweep_cnt, long long *total) {
+  __shared__ int s_cnt[kCompactBlock / 64];
+  __shared__ int s_base;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int t = -1;
   bool f = false;
@@ -831,14 +841,35 @@ __global__ void k_compact(const int *src, int n, int reverse, unsigned char *fla
     f = flag[t] != 0;
     if (f) flag[t] = 0;
   }
-  wave_append(f, t, q.list, q.ctr + QSLOT * q.slot, sweep_cnt, total);
+  const unsigned long long m = __ballot(f);
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  if (lane == 0) s_cnt[w] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sum = 0;
+    for (int k = 0; k < kCompactBlock / 64; ++k) sum += s_cnt[k];
+    int base = 0;
+    if (sum) {
+      base = atomicAdd(q.ctr + QSLOT * q.slot, sum);
+      if (sweep_cnt) atomicAdd(sweep_cnt, sum);
+      if (total) atomicAdd((unsigned long long *)total, (unsigned long long)sum);
+    }
+    s_base = base;
+  }
+  __syncthreads();
+  if (f) {
+    int base = s_base;
+    for (int k = 0; k < w; ++k) base += s_cnt[k];
+    q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+  }
 }
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const Queue &q, int *sweep_cnt, long long *total,
                     hipStream_t st) {
   if (n <= 0) return;
-  k_compact<<<(n + 255) / 256, 256, 0, st>>>(src, n, reverse, flag, q, sweep_cnt, total);
+  k_compact<<<(n + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, st>>>(
+      src, n, reverse, flag, q, sweep_cnt, total);
 }
 
 // ---- persistent tile queue ----------------------------------------------------------
@@ -933,6 +964,7 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
   __shared__ int s_run;
   QueueCursor Qc(q, &s_run);
   const long long st = P.stride;
+  long long my_iters = 0;
   int idx = -1, end = 0;
   while (Qc.next(idx, end)) {
     const int t = q.list[idx];
@@ -949,45 +981,52 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
     // A tile whose every plane is already written in this solve (mat[t])
     // loads cap directly; otherwise the per-plane fresh bits pick c0 or cap.
     const bool fresh_tile = mat[t] == 0;
-    long long gv = 0;
-    int e0 = 0, h0 = HINF;
+
+    // ---- loads: every global load of the visit is issued before any is
+    // used; lanes off the grid, planes d >= K and dead halo cells read a
+    // valid dummy address and discard the value.
+    const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
+    long long ca[S::KMAX];
+#pragma unroll
+    for (int d = 0; d < S::KMAX; ++d) ca[d] = (valid && d < K) ? d * st + gv : 0;
+    HaloCell hc[S::NCELL];
+    int hh[S::NCELL];
+    unsigned hm[S::NCELL];
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+      hh[c] = P.h[hc[c].g];
+      hm[c] = P.rmask[hc[c].g];
+    }
+    int e0 = P.e[gv], h0 = P.h[gv];
     unsigned rm = 0;
     int c0r[S::KMAX];
-    if (valid) gv = node_index(I, gx, gy, gz);
     if (!fresh_tile) {
-      if (valid) {
-        e0 = P.e[gv];
-        h0 = P.h[gv];
-      }
 #pragma unroll
-      for (int d = 0; d < S::KMAX; ++d)
-        c0r[d] = (valid && d < K) ? P.cap[d * st + gv] : 0;
+      for (int d = 0; d < S::KMAX; ++d) c0r[d] = P.cap[ca[d]];
     } else {
-      if (valid) {
-        e0 = P.e[gv];
-        h0 = P.h[gv];
-        rm = P.rmask[gv];
-      }
+      rm = P.rmask[gv];
 #pragma unroll
-      for (int d = 0; d < S::KMAX; ++d)
-        c0r[d] = (valid && d < K) ? ((rm >> (8 + d)) & 1u ? P.c0 : P.cap)[d * st + gv] : 0;
+      for (int d = 0; d < S::KMAX; ++d) c0r[d] = ((rm >> (8 + d)) & 1u ? P.c0 : P.cap)[ca[d]];
     }
+    if (!valid) {
+      e0 = 0;
+      h0 = HINF;
+      rm = 0;
+    }
+#pragma unroll
+    for (int d = 0; d < S::KMAX; ++d)
+      if (!(valid && d < K)) c0r[d] = 0;
+
     s_e[L.me] = e0;
     s_h[L.me] = h0;
 #pragma unroll
     for (int d = 0; d < S::KMAX; ++d) s_c[d][L.me] = c0r[d];
-    for (int p = tid; p < S::PADN; p += S::NT) {
-      int x, y, z, nout;
-      bool inter;
-      pad_coords<S>(p, x, y, z, inter, nout);
-      if (inter) continue;
-      int hv = HINF;
-      if (halo_needed<S>(nout, K)) {
-        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
-          hv = P.h[node_index(I, X, Y, Z)];
-      }
-      s_h[p] = hv;
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      if (!hc[c].halo) continue;
+      const int p = tid + c * S::NT;
+      s_h[p] = hc[c].live ? hh[c] : HINF;
       s_e[p] = 0;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) s_c[d][p] = 0;
@@ -1040,10 +1079,9 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
       }
       any = __syncthreads_or(pushed);
     } while (any && ++it < max_iters);
-    if (tid == 0 && iter_total)
-      atomicAdd((unsigned long long *)iter_total, (unsigned long long)(it + (any ? 0 : 1)));
+    my_iters += it + (any ? 0 : 1);
 
-    // write back what changed
+    // ---- write back what changed (stores only)
     int act = 0;
     if (valid) {
       const int e1 = s_e[L.me], h1 = s_h[L.me];
@@ -1063,52 +1101,54 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
       if (m1 != (fresh_tile ? rm : m0)) P.rmask[gv] = m1;
       act = e1 > 0 && h1 < HINF;
     }
-    // flush flow that crossed into the halo: excess and the reverse residual
-    // of each crossing arc (graphcut_push_relabel.cl:184-195)
-    for (int p = tid; p < S::PADN; p += S::NT) {
-      int x, y, z, nout;
-      bool inter;
-      pad_coords<S>(p, x, y, z, inter, nout);
-      if (inter || !halo_needed<S>(nout, K)) continue;
-      const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-      if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) continue;
-      const long long g = node_index(I, X, Y, Z);
+    // ---- flush flow that crossed into the halo: excess and the reverse
+    // residual of each crossing arc (graphcut_push_relabel.cl:184-195).
+    // Plane d of halo node g is the arc from g into this tile, so no other
+    // tile of this colour class touches plane d of g or its two mask bits in
+    // this launch; other tiles may update other bits of the same mask word,
+    // hence atomics.  A plane that is still fresh (its value lives in c0) is
+    // written as c0 + dc; the others take a fire-and-forget atomic add.
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      if (!hc[c].live) continue;
+      const int p = tid + c * S::NT;
+      const long long g = hc[c].g;
       const int f = s_e[p];
       if (f != 0) {
         atomicAdd(&P.e[g], f);
-        flag[tile_of<S>(I, X, Y, Z)] = 1;
+        flag[tile_of<S>(I, r.ox + (p % S::PW) - 1, r.oy + (p / S::PW) % S::PH - 1,
+                        S::D3 ? r.oz + p / (S::PW * S::PH) - 1 : 0)] = 1;
       }
-      // Reverse residuals of the crossing arcs (> 0 now).  Plane d of halo
-      // node g is the arc from g into this tile, so no other tile of this
-      // colour class touches plane d of g or its two mask bits in this
-      // launch; other tiles may update other bits of the same mask word,
-      // hence the atomic xor of exactly the bits that change.
-      unsigned rg = 0, flip = 0;
-      bool loaded = false;
+      int dc[S::KMAX];
+      unsigned fresh = 0, flip = 0;
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) {
+        dc[d] = d < K ? s_c[d][p] : 0;
+        if (dc[d] != 0) {
+          if ((hm[c] >> (8 + d)) & 1u) fresh |= 1u << d;
+          if (!((hm[c] >> d) & 1u)) flip |= 1u << d;
+        }
+      }
+      flip |= fresh << 8;
+      if (fresh) {
+        int cv[S::KMAX];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) cv[d] = P.c0[((fresh >> d) & 1u) ? d * st + g : 0];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d)
+          if ((fresh >> d) & 1u) P.cap[d * st + g] = cv[d] + dc[d];
+      }
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d)
-        if (d < K) {
-          const int dc = s_c[d][p];
-          if (dc != 0) {
-            if (!loaded) {
-              rg = P.rmask[g];
-              loaded = true;
-            }
-            const long long a = d * st + g;
-            if ((rg >> (8 + d)) & 1u) {
-              P.cap[a] = P.c0[a] + dc;
-              flip |= 1u << (8 + d);
-            } else {
-              P.cap[a] += dc;
-            }
-            if (!((rg >> d) & 1u)) flip |= 1u << d;
-          }
-        }
+        if (dc[d] != 0 && !((fresh >> d) & 1u)) atomicAdd(&P.cap[d * st + g], dc[d]);
       if (flip) atomicXor(&P.rmask[g], flip);
     }
     if (__syncthreads_or(act) && tid == 0) flag[t] = 1;
     if (fresh_tile && tid == 0) mat[t] = 1;
   }
+  // one atomic per workgroup, spread over 64 counters
+  if (tid == 0 && iter_total && my_iters)
+    atomicAdd((unsigned long long *)&iter_total[blockIdx.x & 63], (unsigned long long)my_iters);
 }
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
@@ -1141,34 +1181,31 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     const int K = I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
-    long long gv = 0;
-    int h0 = HINF, e0 = 0;
-    unsigned mask = 0;
-    if (valid) {
-      gv = node_index(I, gx, gy, gz);
-      h0 = P.h[gv];
-      e0 = P.e[gv];
-      mask = P.rmask[gv] & ((1u << K) - 1u);
+    // every load of the visit is issued before any is used (dead lanes and
+    // halo cells read a valid dummy address)
+    const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
+    HaloCell hc[S::NCELL];
+    int hh[S::NCELL];
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+      // device-coherent read: a neighbour tile settled earlier in this same
+      // pass (possibly on another XCD) is seen at once, so one pass carries
+      // the distance front across many tiles in list order.  Any value read
+      // is an upper bound of the distance, so stale is safe.
+      hh[c] = __hip_atomic_load(&P.h[hc[c].g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    int h0 = P.h[gv], e0 = P.e[gv];
+    unsigned mask = P.rmask[gv] & ((1u << K) - 1u);
+    if (!valid) {
+      h0 = HINF;
+      e0 = 0;
+      mask = 0;
     }
     s_h[L.me] = h0;
-    for (int p = tid; p < S::PADN; p += S::NT) {
-      int x, y, z, nout;
-      bool inter;
-      pad_coords<S>(p, x, y, z, inter, nout);
-      if (inter) continue;
-      int hv = HINF;
-      if (halo_needed<S>(nout, K)) {
-        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-        // device-coherent read: a neighbour tile settled earlier in this
-        // same pass (possibly on another XCD) is seen at once, so one pass
-        // carries the distance front across many tiles in list order.  Any
-        // value read is an upper bound of the distance, so stale is safe.
-        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz)
-          hv = __hip_atomic_load(&P.h[node_index(I, X, Y, Z)], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-      }
-      s_h[p] = hv;
-    }
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c)
+      if (hc[c].halo) s_h[tid + c * S::NT] = hc[c].live ? hh[c] : HINF;
     __syncthreads();
     int any;
     do {
@@ -1247,38 +1284,40 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
     const int K = TAG ? (S::D3 ? 6 : 4) : I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
-    long long gv = 0;
-    unsigned char l0 = TAG ? 4 : 0;
-    unsigned mask = 0;
-    bool deficit = false;
-    if (valid) {
-      gv = node_index(I, gx, gy, gz);
-      if (TAG) {
-        l0 = P.lab[gv] ? 1 : object[gv] ? 0 : 4;
-        mask = kAxis;
-      } else {
-        l0 = P.lab[gv];
-        deficit = P.e[gv] < 0;
-        mask = P.rmask[gv] & ((1u << K) - 1u);
-        if (l0 && deficit) *cert = 1;
-      }
+    // every load of the visit is issued before any is used (dead lanes and
+    // halo cells read a valid dummy address)
+    const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
+    HaloCell hc[S::NCELL];
+    unsigned char hl[S::NCELL], ho[S::NCELL];
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+      hl[c] = P.lab[hc[c].g];
+      if (TAG) ho[c] = object[hc[c].g];
     }
+    unsigned char l0 = P.lab[gv];
+    unsigned mask;
+    bool deficit = false;
+    if (TAG) {
+      l0 = l0 ? 1 : object[gv] ? 0 : 4;
+      mask = kAxis;
+    } else {
+      deficit = P.e[gv] < 0;
+      mask = P.rmask[gv] & ((1u << K) - 1u);
+    }
+    if (!valid) {
+      l0 = TAG ? 4 : 0;
+      mask = 0;
+      deficit = false;
+    }
+    if (!TAG && l0 && deficit) *cert = 1;
     s_l[L.me] = l0;
-    for (int p = tid; p < S::PADN; p += S::NT) {
-      int x, y, z, nout;
-      bool inter;
-      pad_coords<S>(p, x, y, z, inter, nout);
-      if (inter) continue;
+#pragma unroll
+    for (int c = 0; c < S::NCELL; ++c) {
+      if (!hc[c].halo) continue;
       unsigned char lv = 1;
-      if (halo_needed<S>(nout, K)) {
-        const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-        if (X >= 0 && X < I.nx && Y >= 0 && Y < I.ny && Z >= 0 && Z < I.nz) {
-          const long long hv = node_index(I, X, Y, Z);
-          lv = P.lab[hv];
-          if (TAG) lv = lv ? 1 : object[hv] ? 0 : 4;
-        }
-      }
-      s_l[p] = lv;
+      if (hc[c].live) lv = TAG ? (hl[c] ? 1 : ho[c] ? 0 : 4) : hl[c];
+      s_l[tid + c * S::NT] = lv;
     }
     __syncthreads();
     int any;
