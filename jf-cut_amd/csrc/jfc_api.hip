@@ -168,26 +168,29 @@ struct jfc_graph {
     int tw, th, td;
     shape_dims(sh, &tw, &th, &td);
     bool all4 = true;
-    int off = 0;
+    long long off = 0;
     max_tile_dim = 1;
     int maxsum = 0;
     for (auto &I : inst) {
       I.tx = (I.nx + tw - 1) / tw;
       I.ty = (I.ny + th - 1) / th;
       I.tz = (I.nz + td - 1) / td;
-      I.tile_off = off;
-      I.ntiles = I.tx * I.ty * I.tz;
-      off += I.ntiles;
+      const long long nt = (long long)I.tx * I.ty * I.tz;
+      if (off + nt >= kEntryMat) return fail(JFC_ERR_INVALID, "graph too large for the tile queue");
+      I.tile_off = (int)off;
+      I.ntiles = (int)nt;
+      off += nt;
       if (I.K != 4) all4 = false;
       max_tile_dim = std::max(max_tile_dim, std::max(I.tx, std::max(I.ty, I.tz)));
       maxsum = std::max(maxsum, I.tx + I.ty + I.tz);
     }
-    ntiles = off;
+    ntiles = (int)off;
     // distance bins of the tile ordering (the last bin holds tiles of
     // instances without seed tiles); farther tiles share the last finite bin
     nbins = std::min(maxsum + 2, 4096);
-    // colour classes: tiles of one class share no node or halo cell.
-    // 2D 4C and 3D 6C: checkerboard; 2D 8C: 2x2 pattern.
+    // colour classes: tiles of one class are never adjacent through an arc,
+    // so no node is in one tile of the class and in the halo of another it
+    // pushes across.  2D 4C and 3D 6C: checkerboard; 2D 8C: 2x2 pattern.
     ncolors = (dim == 2 && !all4) ? 4 : 2;
     std::vector<int> tile_inst(ntiles);
     std::vector<std::vector<int>> cls(ncolors);
@@ -591,7 +594,7 @@ struct Solver {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
-      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, q, nullptr, total, st);
+      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, q, nullptr, total, st);
       flood(q);
     } else {
       flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles});
@@ -603,7 +606,8 @@ struct Solver {
     for (int pass = 1;;) {
       for (int k = 0; k < kPassGroup; ++k, ++pass) {
         Queue qc = queue_from_list();
-        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, qc, nullptr, total, st);
+        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, qc, nullptr, total,
+                       st);
         flood(qc);
         advance();
         *passes += 1;
@@ -707,7 +711,7 @@ struct Solver {
         for (int c = 0; c < g->ncolors; ++c) {
           Queue q = queue_from_list();
           const int n = g->color_off[c + 1] - g->color_off[c];
-          launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, q,
+          launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, q,
                          g->d_sweep + sweep, g->d_total, st);
           hipEvent_t a = o.profile ? tm.mark() : nullptr;
           launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,

@@ -104,9 +104,12 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *order, int max_tile_dim, int *max_dist,
                        hipStream_t st);
 
+// Queue list entries: tile index, plus kEntryMat when compacted with `mat`
+// and mat[t] is set.
+constexpr int kEntryMat = 1 << 30;
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
-                    const Queue &q, int *sweep_cnt, long long *total,
-                    hipStream_t st);
+                    const unsigned char *mat, const Queue &q, int *sweep_cnt,
+                    long long *total, hipStream_t st);
 void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
                        hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in

@@ -822,14 +822,17 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
 // ---- list compaction (ballots + one atomic per 1024-lane block) ------------------
 //
 // Flagged entries of `src` (in order, or reversed) are appended to the queue
-// list and their flags cleared.  Each wave counts its hits with a ballot; the
+// list and their flags cleared; with `mat`, entries of tiles whose residual
+// planes are already written carry kEntryMat, so the push kernel knows it
+// without a dependent load.  Each wave counts its hits with a ballot; the
 // block adds its total to the list count (and to the sweep and run totals)
 // with one atomic each, so a pass over 10^5..10^6 tiles costs a few hundred
 // same-address atomics.
 constexpr int kCompactBlock = 1024;
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n, int reverse,
-                                                           unsigned char *flag, Queue q,
+                                                           unsigned char *flag,
+                                                           const unsigned char *mat, Queue q,
                                                            int *sweep_cnt, long long *total) {
   __shared__ int s_cnt[kCompactBlock / 64];
   __shared__ int s_base;
@@ -860,16 +863,16 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
   if (f) {
     int base = s_base;
     for (int k = 0; k < w; ++k) base += s_cnt[k];
-    q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+    q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = t | (mat && mat[t] ? kEntryMat : 0);
   }
 }
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
-                    const Queue &q, int *sweep_cnt, long long *total,
-                    hipStream_t st) {
+                    const unsigned char *mat, const Queue &q, int *sweep_cnt,
+                    long long *total, hipStream_t st) {
   if (n <= 0) return;
   k_compact<<<(n + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, st>>>(
-      src, n, reverse, flag, q, sweep_cnt, total);
+      src, n, reverse, flag, mat, q, sweep_cnt, total);
 }
 
 // ---- persistent tile queue ----------------------------------------------------------
@@ -890,6 +893,7 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // next producer.
 struct QueueCursor {
   int count, chunk, first, head_id, nheads;
+  int ahead;   // (lane 0) claim for the run after the current one
   int *heads;
   int *s_run;  // LDS word broadcasting the claimed run
   __device__ __forceinline__ QueueCursor(const Queue &q, int *lds_word) {
@@ -909,8 +913,11 @@ struct QueueCursor {
     }
   }
   // Next list index for the whole workgroup (uniform); false once the list is
-  // drained.  Start with idx = -1, end = 0.  Between two calls the caller
-  // passes at least one barrier, so s_run is not overwritten while read.
+  // drained.  Start with idx = -1, end = 0.  The claim for the following run
+  // is issued when a run starts and read when it ends, so its latency hides
+  // behind the run's tiles; every claimed run is processed at the next call
+  // or lies past the end.  Between two calls the caller passes at least one
+  // barrier, so s_run is not overwritten while read.
   __device__ __forceinline__ bool next(int &idx, int &end) {
     if (idx + 1 < end) {
       ++idx;
@@ -918,16 +925,39 @@ struct QueueCursor {
     }
     int run = first;
     if (idx >= 0) {
-      if (threadIdx.x == 0) *s_run = atomicAdd(&heads[head_id], 1);
+      if (threadIdx.x == 0) *s_run = ahead;
       __syncthreads();
       const int k = *s_run;  // k-th run handed out by this head
       run = (int)gridDim.x + 8 * (head_id + nheads * (k >> 3)) + (k & 7);
     }
     idx = run * chunk;
     end = min(idx + chunk, count);
-    return idx < count;
+    const bool ok = idx < count;
+    if (ok && threadIdx.x == 0) ahead = atomicAdd(&heads[head_id], 1);
+    return ok;
   }
 };
+
+// Grid of a persistent (queue-consuming) launch: every workgroup must be
+// resident from the start -- one dispatched late finds the queue drained but
+// still runs its static first run after the others are done, stretching the
+// launch -- so the grid is capped at what the device holds at once for this
+// kernel (occupancy per CU, from its register and LDS use, times CUs).
+template <auto Kernel>
+static int resident_grid(int want) {
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, Kernel, 256, 0) != hipSuccess || n < 1)
+      n = 1;
+    return n;
+  }();
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  return std::max(1, std::min(want, per_cu * cus));
+}
 
 // ---- residual-arc masks and lazy residual planes ----------------------------------
 //
@@ -967,7 +997,8 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
   long long my_iters = 0;
   int idx = -1, end = 0;
   while (Qc.next(idx, end)) {
-    const int t = q.list[idx];
+    const int ent = q.list[idx];
+    const int t = ent & ~kEntryMat;
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
@@ -978,9 +1009,10 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
     // (the role of cl_max_height in graphcut_push_relabel.cl, made exact).
     const long long nn = (long long)I.nx * I.ny * I.nz;
     const int hcap = nn < HINF ? (int)nn : HINF;
-    // A tile whose every plane is already written in this solve (mat[t])
-    // loads cap directly; otherwise the per-plane fresh bits pick c0 or cap.
-    const bool fresh_tile = mat[t] == 0;
+    // A tile whose every plane is already written in this solve (kEntryMat,
+    // from mat[t]) loads cap directly; otherwise the per-plane fresh bits
+    // pick c0 or cap.
+    const bool fresh_tile = (ent & kEntryMat) == 0;
 
     // ---- loads: every global load of the visit is issued before any is
     // used; lanes off the grid, planes d >= K and dead halo cells read a
@@ -1154,8 +1186,8 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
                  int max_iters, long long *iter_total, hipStream_t st) {
-  JFC_DISPATCH(shape,
-               (k_push<S><<<nwg, 256, 0, st>>>(T, P, q, flag, mat, max_iters, iter_total)));
+  JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
+                          T, P, q, flag, mat, max_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
@@ -1250,7 +1282,8 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
                   hipStream_t st) {
-  JFC_DISPATCH(shape, (k_relax<S><<<nwg, 256, 0, st>>>(T, P, q, dirty, active)));
+  JFC_DISPATCH(shape, (k_relax<S><<<resident_grid<&k_relax<S>>(nwg), 256, 0, st>>>(
+                          T, P, q, dirty, active)));
 }
 
 // ---- canonical cut: residual reachability from excess nodes -------------------------
@@ -1358,13 +1391,15 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st) {
-  JFC_DISPATCH(shape, (k_reach<S, false><<<nwg, 256, 0, st>>>(T, P, q, dirty, cert, nullptr)));
+  JFC_DISPATCH(shape, (k_reach<S, false><<<resident_grid<&k_reach<S, false>>(nwg), 256, 0, st>>>(
+                          T, P, q, dirty, cert, nullptr)));
 }
 
 void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
                 const Queue &q, unsigned char *dirty, const unsigned char *object,
                 hipStream_t st) {
-  JFC_DISPATCH(shape, (k_reach<S, true><<<nwg, 256, 0, st>>>(T, P, q, dirty, nullptr, object)));
+  JFC_DISPATCH(shape, (k_reach<S, true><<<resident_grid<&k_reach<S, true>>(nwg), 256, 0, st>>>(
+                          T, P, q, dirty, nullptr, object)));
 }
 
 }  // namespace jfc
