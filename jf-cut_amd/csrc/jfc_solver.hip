@@ -176,6 +176,19 @@ __device__ __forceinline__ int block_or_bits(int bits, int *s_w, int parity) {
   return buf[0] | buf[1] | buf[2] | buf[3];
 }
 
+// Any lane of the 256-lane workgroup with pred set: one ballot per wave, one
+// LDS word per wave, one barrier.  The buffer half alternates with `parity`
+// (as in block_or_bits), so a second barrier is not needed before reuse.
+// The barrier also orders the workgroup's LDS accesses like __syncthreads.
+__device__ __forceinline__ int block_any(int pred, int *s_w, int &parity) {
+  const int w = __ballot(pred) != 0;
+  int *buf = s_w + 4 * parity;
+  parity ^= 1;
+  if (lane_id() == 0) buf[threadIdx.x >> 6] = w;
+  __syncthreads();
+  return buf[0] | buf[1] | buf[2] | buf[3];
+}
+
 // Node of lane L in tile t; inst = -1 past the end of the chunk.
 struct TileNode {
   int inst;
@@ -983,7 +996,8 @@ static int resident_grid(int want) {
 // whole iteration pushes nothing (graphcut_push_relabel.cl:77-182) or after
 // max_iters iterations.
 template <class S>
-__global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
+// 5 waves per SIMD: the register budget that keeps the load phase unspilled.
+__global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
                                               int max_iters, long long *iter_total) {
   __shared__ int s_h[S::PADN];
@@ -992,6 +1006,8 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   __shared__ int s_run;
+  __shared__ int s_w[8];
+  int par = 0;
   QueueCursor Qc(q, &s_run);
   const long long st = P.stride;
   long long my_iters = 0;
@@ -1109,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
           }
         }
       }
-      any = __syncthreads_or(pushed);
+      any = block_any(pushed, s_w, par);
     } while (any && ++it < max_iters);
     my_iters += it + (any ? 0 : 1);
 
@@ -1175,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_push(Tiles T, Planes P, Queue q,
         if (dc[d] != 0 && !((fresh >> d) & 1u)) atomicAdd(&P.cap[d * st + g], dc[d]);
       if (flip) atomicXor(&P.rmask[g], flip);
     }
-    if (__syncthreads_or(act) && tid == 0) flag[t] = 1;
+    if (block_any(act, s_w, par) && tid == 0) flag[t] = 1;
     if (fresh_tile && tid == 0) mat[t] = 1;
   }
   // one atomic per workgroup, spread over 64 counters
@@ -1204,6 +1220,8 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   __shared__ int s_run;
+  __shared__ int s_w[8];
+  int par = 0;
   QueueCursor Qc(q, &s_run);
   int idx = -1, end = 0;
   while (Qc.next(idx, end)) {
@@ -1253,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
           ch = 1;
         }
       }
-      any = __syncthreads_or(ch);
+      any = block_any(ch, s_w, par);
     } while (any);
     if (valid) {
       const int h1 = s_h[L.me];
@@ -1274,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     }
     // graphcut_count_node fused in: the tile's last visit leaves its
     // active flag (a node with excess and a finite height)
-    const int act = __syncthreads_or(valid && e0 > 0 && s_h[L.me] < HINF);
+    const int act = block_any(valid && e0 > 0 && s_h[L.me] < HINF, s_w, par);
     if (tid == 0) active[t] = (unsigned char)act;
   }
 }
@@ -1307,6 +1325,8 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   __shared__ int s_run;
+  __shared__ int s_w[8];
+  int par = 0;
   QueueCursor Qc(q, &s_run);
   constexpr unsigned kAxis = S::D3 ? 0x3fu : 0xfu;
   int idx = -1, end = 0;
@@ -1368,7 +1388,7 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
             }
           }
       }
-      any = __syncthreads_or(ch);
+      any = block_any(ch, s_w, par);
     } while (any);
     if (valid && s_l[L.me] == 2) {
       P.lab[gv] = 1;
