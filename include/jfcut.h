@@ -77,6 +77,10 @@ typedef struct {
   double stall_ratio;     /* run a global relabel when the active-tile count of a
                              sweep stays within [r, 1/r] of the previous sweep's
                              (the active set has stopped changing; 0 = off) */
+  int graph_sweeps;       /* 1 = replay each sweep's launches (compaction and
+                             push per colour class) as one captured hipGraph
+                             (default); 0 = launch them one by one.  profile = 1
+                             always launches them one by one. */
 } jfc_options;
 
 #define JFC_TILE_DEFAULT 0

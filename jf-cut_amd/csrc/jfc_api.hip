@@ -134,10 +134,28 @@ struct jfc_graph {
   std::vector<long long> flows;
   std::vector<hipEvent_t> ev_pool;
   int nwg = 2048;
+  int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
+  hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
+  hipEvent_t priv_event = nullptr;
+  // Captured launch sequence of one sweep, per starting queue slot.
+  struct SweepGraph {
+    hipGraphExec_t exec = nullptr;
+    hipStream_t st = nullptr;
+    int mli = 0;
+    const int *hist = nullptr;
+  } sweep_graph[2];
+
+  void drop_graphs() {
+    for (auto &sg : sweep_graph) {
+      if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+      sg = SweepGraph{};
+    }
+  }
 
   ~jfc_graph() { release(); }
 
   void release_tiles() {
+    drop_graphs();
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
@@ -146,7 +164,11 @@ struct jfc_graph {
     release_tiles();
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
     dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
-    dfree(d_def); dfree(d_row_off);
+    dfree(d_def); dfree(d_row_off); dfree(d_sweep_state);
+    if (priv_event) (void)hipEventDestroy(priv_event);
+    if (priv_stream) (void)hipStreamDestroy(priv_stream);
+    priv_event = nullptr;
+    priv_stream = nullptr;
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
@@ -276,6 +298,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->stream = nullptr;
   o->initial_relabel = 0;
   o->stall_ratio = 0.99;
+  o->graph_sweeps = 1;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -328,6 +351,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
       (rc = dalloc(&g->d_total, kTotals, "counters")) ||
       (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
+      (rc = dalloc(&g->d_sweep_state, 2, "sweep counters")) ||
       (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
     delete g;
@@ -658,12 +682,64 @@ struct Solver {
     return read_int(g->d_flagword, found);
   }
 
+  // One push-relabel sweep: per colour class, compact the flagged tiles and
+  // discharge them; then file the sweep's active-tile count.  The sequence is
+  // the same for every sweep that starts on the same queue slot (the colour
+  // count is even), which is what lets replay_sweep() capture it once.
+  void issue_sweep(hipStream_t s_, bool timed) {
+    const Tiles T = g->tiles();
+    const Planes P = g->planes();
+    for (int c = 0; c < g->ncolors; ++c) {
+      Queue q = queue_from_list();
+      const int n = g->color_off[c + 1] - g->color_off[c];
+      launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, q,
+                     g->d_sweep_state, g->d_total, s_);
+      hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
+      launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
+                  o.max_local_iters, g->d_total + kIterSlots, s_);
+      if (timed && o.profile) push_ev.push_back({a, tm.mark()});
+      advance();
+    }
+    launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, s_);
+  }
+
+  // Launch one sweep as a hipGraph captured on first use (per starting
+  // slot, stream and local-iteration cap; tile-table or counter changes drop
+  // the captured graphs).  The sweep contains no memset node.
+  int replay_sweep() {
+    auto &sg = g->sweep_graph[g->slot];
+    if (!sg.exec || sg.st != st || sg.mli != o.max_local_iters || sg.hist != g->d_sweep) {
+      if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+      sg = jfc_graph::SweepGraph{};
+      const int slot0 = g->slot;
+      hipGraph_t graph = nullptr;
+      JFC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      issue_sweep(st, false);
+      const hipError_t ec = hipStreamEndCapture(st, &graph);
+      g->slot = slot0;
+      if (ec != hipSuccess) return fail(JFC_ERR_DEVICE, std::string("sweep capture: ") + hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        sg.exec = nullptr;
+        return fail(JFC_ERR_DEVICE, std::string("sweep graph: ") + hipGetErrorString(ei));
+      }
+      sg.st = st;
+      sg.mli = o.max_local_iters;
+      sg.hist = g->d_sweep;
+    }
+    JFC_HIP(hipGraphLaunch(sg.exec, st));
+    for (int c = 0; c < g->ncolors; ++c) advance();
+    return JFC_OK;
+  }
+
   int run() {
     const Tiles T = g->tiles();
     const Planes P = g->planes();
     const int nI = (int)g->inst.size();
     // reset the residual state from the inputs
     if (g->sweep_cap < o.max_sweeps + 1) {
+      g->drop_graphs();
       dfree(g->d_sweep);
       int rc = dalloc(&g->d_sweep, (size_t)o.max_sweeps + 1, "sweep counters");
       if (rc) return rc;
@@ -674,6 +750,7 @@ struct Solver {
     launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
     launch_fill_i32((int *)g->d_total, 2 * kTotals, 0, st);
     launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
+    launch_fill_i32(g->d_sweep_state, 2, 0, st);
     g->slot = 0;
     // the residual-arc masks restart from the inputs' with every plane
     // marked fresh: k_push reads fresh planes from c0 and writes them to cap
@@ -708,18 +785,14 @@ struct Solver {
     while (!done) {
       bool active = true;
       while (sweep < o.max_sweeps) {
-        for (int c = 0; c < g->ncolors; ++c) {
-          Queue q = queue_from_list();
-          const int n = g->color_off[c + 1] - g->color_off[c];
-          launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, q,
-                         g->d_sweep + sweep, g->d_total, st);
-          hipEvent_t a = o.profile ? tm.mark() : nullptr;
-          launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag,
-                      g->d_mat, o.max_local_iters, g->d_total + kIterSlots, st);
-          if (o.profile) push_ev.push_back({a, tm.mark()});
-          advance();
-          s.push_launches++;
+        // the first sweep of a solve runs eagerly (it also primes the
+        // launchers' occupancy queries); later ones replay a captured graph
+        if (o.graph_sweeps && !o.profile && sweep > 0) {
+          if ((rc = replay_sweep())) return rc;
+        } else {
+          issue_sweep(st, true);
         }
+        s.push_launches += g->ncolors;
         ++sweep;
         if (++since_check >= o.check_every || sweep == o.max_sweeps) {
           since_check = 0;
@@ -814,6 +887,20 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (rc) return rc;
   std::memset(&S.s, 0, sizeof(S.s));
   S.st = (hipStream_t)S.o.stream;
+  if (!S.st) {
+    // the legacy default stream cannot be captured: solve on a stream of the
+    // graph's own, ordered after the work already queued on the default one
+    if (!g->priv_stream) {
+      if (hipStreamCreateWithFlags(&g->priv_stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&g->priv_event, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(JFC_ERR_DEVICE, "cannot create the solve stream");
+      }
+    }
+    JFC_HIP(hipEventRecord(g->priv_event, nullptr));
+    JFC_HIP(hipStreamWaitEvent(g->priv_stream, g->priv_event, 0));
+    S.st = g->priv_stream;
+  }
   S.tm.pool = &g->ev_pool;
   S.tm.st = S.st;
   g->solved = false;

@@ -104,6 +104,10 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *order, int max_tile_dim, int *max_dist,
                        hipStream_t st);
 
+// state = {count of the running sweep, sweep index}: hist[index] = count,
+// then the next sweep starts at 0.
+void launch_sweep_end(int *hist, int *state, int cap, hipStream_t st);
+
 // Queue list entries: tile index, plus kEntryMat when compacted with `mat`
 // and mat[t] is set.
 constexpr int kEntryMat = 1 << 30;

@@ -880,6 +880,19 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
   }
 }
 
+// End of a sweep: file the sweep's active-tile count (accumulated by the
+// compactions in state[0]) under the sweep index state[1] and start the next.
+__global__ void k_sweep_end(int *hist, int *state, int cap) {
+  const int i = state[1];
+  if (i < cap) hist[i] = state[0];
+  state[0] = 0;
+  state[1] = i + 1;
+}
+
+void launch_sweep_end(int *hist, int *state, int cap, hipStream_t st) {
+  k_sweep_end<<<1, 1, 0, st>>>(hist, state, cap);
+}
+
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const Queue &q, int *sweep_cnt,
                     long long *total, hipStream_t st) {

@@ -36,7 +36,7 @@ class Options(ctypes.Structure):
                 ("relabel_work", ctypes.c_double), ("max_local_iters", ctypes.c_int),
                 ("tile_shape", ctypes.c_int), ("profile", ctypes.c_int),
                 ("stream", ctypes.c_void_p), ("initial_relabel", ctypes.c_int),
-                ("stall_ratio", ctypes.c_double)]
+                ("stall_ratio", ctypes.c_double), ("graph_sweeps", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

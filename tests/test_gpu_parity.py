@@ -1,5 +1,6 @@
 """GPU parity: the HIP solver (through the C-ABI) against the oracle, BK303 and
 the committed golden fixtures.  Bit-exact: int64 flow and canonical labels."""
+import ctypes
 import hashlib
 import sys
 
@@ -65,7 +66,8 @@ def test_random_grids_gpu(gpu, conn, shape):
 
 @pytest.mark.parametrize("opts", [dict(tile_shape="64x4"), dict(tile_shape="16x16"),
                                   dict(max_local_iters=1), dict(relabel_work=0.0),
-                                  dict(check_every=1, max_local_iters=1000)])
+                                  dict(check_every=1, max_local_iters=1000),
+                                  dict(graph_sweeps=0), dict(check_every=3)])
 def test_options_do_not_change_result(gpu, opts):
     rng = np.random.default_rng(99)
     nx, ny = 203, 147
@@ -77,6 +79,36 @@ def test_options_do_not_change_result(gpu, opts):
         assert g.flow() == f
         assert np.array_equal(g.labels(), l)
         g.close()
+
+
+def test_sweep_graph_replays(gpu):
+    """The captured per-sweep hipGraph (default), eager launches and profiled
+    launches give identical flows and labels, over many replays on one graph,
+    with eager work and other option sets in between, and on a caller stream."""
+    import torch
+    rng = np.random.default_rng(31)
+    nx, ny, conn = 190, 170, 8
+    cs, ct, caps = random_grid(rng, nx, ny, 1, conn)
+    f, l, ok = ol.maxflow(nx, ny, 1, conn, cs, ct, caps)
+    g = gpu.GridGraph([(nx, ny, 1, conn)])
+    g.set_terminal_caps(cs, ct)
+    for d in range(conn):
+        g.set_neighbor_caps(d, caps[d])
+    runs = [dict(), dict(graph_sweeps=0), dict(), dict(profile=1), dict(max_local_iters=3), dict(),
+            dict(max_local_iters=3), dict(graph_sweeps=0, max_local_iters=1), dict()]
+    for i, opts in enumerate(runs):
+        say("case replay", i, opts)
+        st = g.solve(**opts)
+        assert st.converged == 1 and st.sweeps >= 2
+        assert g.flow() == f, (i, opts)
+        assert np.array_equal(g.labels(), l), (i, opts)
+        assert g.energy() == f
+    s = torch.cuda.Stream()
+    for i in range(3):
+        say("case stream", i)
+        st = g.solve(stream=s.cuda_stream)
+        assert g.flow() == f and np.array_equal(g.labels(), l)
+    g.close()
 
 
 def test_3d_tile_shape_option(gpu):
