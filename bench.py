@@ -92,6 +92,8 @@ def batched_forward_backward(n=256, reps=3):
     gb = [DeviceBuffer(ntot * 4), DeviceBuffer(ntot * 4), DeviceBuffer(ntot * kmax * 4)]
     maxn = max(s["W"] * s["H"] for s in specs)
     gs = [DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 8 * 4)]
+    # tile shape per size class, chosen by the autotuner (outside the timing)
+    tile = G.autotune(reps=2)
     G.solve()
     for g in singles:
         g.solve()
@@ -123,7 +125,7 @@ def batched_forward_backward(n=256, reps=3):
             "speedup": round(t_seq / t_batch, 2),
             "batched_mvoxels_per_s": round(ntot / t_batch / 1e6, 1),
             "batched_ms": round(t_batch * 1e3, 2), "sequential_ms": round(t_seq * 1e3, 2),
-            "batched_sweeps": st.sweeps, "flows_equal": True}
+            "batched_sweeps": st.sweeps, "tile_shape": tile, "flows_equal": True}
 
 
 def cpu_baseline(wl, sample_w, sample_h):

@@ -219,6 +219,22 @@ int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_sink,
 /* Total nodes of the graph (all instances) and the largest connectivity. */
 int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
 
+/* ---- tile-shape autotuning (no reference counterpart: the reference takes
+ * the tile size from the UI, QGCW:826-835) ---------------------------------- */
+
+/* Time `reps` solves of the graph (after one warm-up) with every tile shape of
+ * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2) under the other
+ * options of opt (may be NULL), and keep the fastest as the shape the graph's
+ * solves use when opt->tile_shape is JFC_TILE_DEFAULT.  The choice is also
+ * stored for the graph's size class (dimensionality, connectivities, instance
+ * count and mean instance size, each rounded to a power of two): graphs of the
+ * class created later in the process start with it.  Results are unchanged by
+ * the shape.  *chosen (may be NULL) receives the JFC_TILE_* value. */
+int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int *chosen);
+
+/* The JFC_TILE_* shape a solve with tile_shape = JFC_TILE_DEFAULT uses. */
+int jfc_get_tile_shape(const jfc_graph *g, int *shape);
+
 /* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
 
 typedef struct {

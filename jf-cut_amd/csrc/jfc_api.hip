@@ -17,8 +17,11 @@
 #include "jfc_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,6 +85,12 @@ int shape_from_option(int opt, int dim) {
   }
 }
 
+// Process-wide autotuning table: size class -> JFC_TILE_* shape.
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;
+
+int log2_round(double v) { return v <= 1.0 ? 0 : (int)std::lround(std::log2(v)); }
+
 // Device counters (int64): [0] push tile visits, [1] relabel tile visits,
 // [2] cut tile visits, [4] tag tile visits, [kIterSlots, +64) local
 // iterations of the push kernel (workgroup b adds to slot b % 64, so no
@@ -96,6 +105,8 @@ long long push_bytes_per_node(int K) { return 2LL * 4 * (2 + K); }
 
 struct jfc_graph {
   int dim = 2;
+  int tuned = 0;           // JFC_TILE_* chosen by jfc_autotune (0 = none)
+  std::string size_class;  // key of the process-wide autotuning table
   int Kg = 4;  // planes stored (max K over instances)
   int shape = 0;
   long long Ntot = 0;
@@ -381,6 +392,16 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
     g->nwg = std::max(64, prop.multiProcessorCount * 8);
+  {
+    unsigned conns = 0;
+    for (const auto &I : g->inst) conns |= 1u << I.conn;
+    const double mean = (double)off / count;
+    g->size_class = std::to_string(g->dim) + ":" + std::to_string(conns) + ":" +
+                    std::to_string(log2_round(count)) + ":" + std::to_string(log2_round(mean));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(g->size_class);
+    if (it != g_tune.end()) g->tuned = it->second;
+  }
   if ((rc = g->set_shape(default_shape(g->dim)))) { delete g; return rc; }
   if (hipDeviceSynchronize() != hipSuccess) {
     delete g;
@@ -881,7 +902,8 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.max_sweeps > (1 << 24)) S.o.max_sweeps = 1 << 24;
   if (S.o.check_every <= 0) S.o.check_every = 1;
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = 1;
-  const int sh = shape_from_option(S.o.tile_shape, g->dim);
+  const int sh = shape_from_option(
+      S.o.tile_shape == JFC_TILE_DEFAULT && g->tuned ? g->tuned : S.o.tile_shape, g->dim);
   if (!sh) return fail(JFC_ERR_INVALID, "tile shape does not match the graph's dimensionality");
   int rc = g->set_shape(sh);
   if (rc) return rc;
@@ -1043,6 +1065,49 @@ extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
   JFC_HIP(hipDeviceSynchronize());
   JFC_HIP(hipMemcpy(flags, src, g->ntiles, hipMemcpyDeviceToHost));
   g->solved = false;
+  return JFC_OK;
+}
+
+extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int *chosen) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  if (reps < 1) reps = 1;
+  jfc_options o;
+  if (opt) o = *opt;
+  else jfc_default_options(&o);
+  const int cand2[3] = {JFC_TILE_32x8, JFC_TILE_64x4, JFC_TILE_16x16};
+  const int cand3[2] = {JFC_TILE_8x8x4, JFC_TILE_32x4x2};
+  const int *cand = g->dim == 3 ? cand3 : cand2;
+  const int nc = g->dim == 3 ? 2 : 3;
+  int best = 0;
+  double best_ms = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    o.tile_shape = cand[c];
+    std::vector<double> ms;
+    for (int r = 0; r <= reps; ++r) {
+      jfc_stats st;
+      const int rc = jfc_solve(g, &o, &st);
+      if (rc < 0) return rc;
+      if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+    }
+    std::sort(ms.begin(), ms.end());
+    const double med = ms[ms.size() / 2];
+    if (!best || med < best_ms) {
+      best = cand[c];
+      best_ms = med;
+    }
+  }
+  g->tuned = best;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[g->size_class] = best;
+  }
+  if (chosen) *chosen = best;
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_tile_shape(const jfc_graph *g, int *shape) {
+  if (!g || !shape) return fail(JFC_ERR_INVALID, "null argument");
+  *shape = g->tuned ? g->tuned : (g->dim == 3 ? JFC_TILE_8x8x4 : JFC_TILE_32x8);
   return JFC_OK;
 }
 

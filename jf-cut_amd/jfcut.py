@@ -24,6 +24,7 @@ ERR_INVALID, ERR_DEVICE, ERR_OOM, ERR_STATE, ERR_OVERFLOW, ERR_NO_SEEDS = -1, -2
 MAX_TERMINAL_CAP = 1 << 30
 MAX_NEIGHBOR_CAP = 1 << 26
 TILE = {"default": 0, "32x8": 1, "64x4": 2, "16x16": 3, "8x8x4": 4, "32x4x2": 5}
+TILE_NAME = {v: k for k, v in TILE.items()}
 
 # direction d -> (dx, dy, dz); reverse of d is d ^ 1
 DIRS_2D = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0),
@@ -84,7 +85,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_debug_tile_order", "jfc_debug_sweep_counts", "jfc_backward_all",
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
-           "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags"]
+           "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
+           "jfc_autotune", "jfc_get_tile_shape"]
 
 
 def lib():
@@ -138,6 +140,8 @@ def lib():
         "jfc_node_write": (I, [ctypes.c_char_p, I, I, I, I, I, I, P, P, P]),
         "jfc_mfi_export": (I, [ctypes.c_char_p, I, I, I, P, P, P, P, P, P, I64]),
         "jfc_get_tags": (I, [P, I, P, P, I]),
+        "jfc_autotune": (I, [P, ctypes.POINTER(Options), I, ctypes.POINTER(I)]),
+        "jfc_get_tile_shape": (I, [P, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -304,6 +308,20 @@ class GridGraph:
         t = np.zeros(f.size, np.uint8)
         _check(lib().jfc_get_tags(self._h, inst, _vp(f), _vp(t), 0))
         return t
+
+    def autotune(self, reps=2, **kw):
+        """Time every tile shape on this graph and keep the fastest
+        (jfc_autotune); returns its name."""
+        o = default_options(**kw)
+        ch = ctypes.c_int()
+        _check(lib().jfc_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(ch)))
+        return TILE_NAME[ch.value]
+
+    def tile_shape(self):
+        """Name of the tile shape default solves of this graph use."""
+        ch = ctypes.c_int()
+        _check(lib().jfc_get_tile_shape(self._h, ctypes.byref(ch)))
+        return TILE_NAME[ch.value]
 
     def size(self):
         """(total nodes over all instances, largest connectivity)."""

@@ -166,6 +166,37 @@ def test_batch_gpu(gpu):
     g.close()
 
 
+def test_autotune_gpu(gpu):
+    """jfc_autotune picks one of the shapes, leaves results unchanged, and a
+    later graph of the same size class starts with the same shape."""
+    rng = np.random.default_rng(77)
+    shapes = [(int(rng.integers(40, 120)), int(rng.integers(40, 120)), 1, 4 if i % 2 else 8)
+              for i in range(6)]
+    data = [random_grid(rng, s[0], s[1], 1, s[3]) for s in shapes]
+
+    def build():
+        g = gpu.GridGraph(shapes)
+        for i, (cs, ct, caps) in enumerate(data):
+            g.set_terminal_caps(cs, ct, inst=i)
+            for d in range(shapes[i][3]):
+                g.set_neighbor_caps(d, caps[d], inst=i)
+        return g
+
+    g = build()
+    best = g.autotune(reps=1)
+    say("case autotune", best)
+    assert best in ("32x8", "64x4", "16x16") and g.tile_shape() == best
+    st = g.solve()
+    assert st.converged == 1
+    for i, (cs, ct, caps) in enumerate(data):
+        f, l, ok = ol.maxflow(shapes[i][0], shapes[i][1], 1, shapes[i][3], cs, ct, caps)
+        assert g.flow(i) == f and np.array_equal(g.labels(i), l), i
+    g.close()
+    g2 = build()
+    assert g2.tile_shape() == best
+    g2.close()
+
+
 def test_backward_gpu(gpu):
     rng = np.random.default_rng(21)
     nx, ny, conn = 40, 30, 8
