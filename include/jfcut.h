@@ -266,6 +266,55 @@ int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
                     int bytes_per_channel, const uint8_t *seeds, int on_device,
                     const jfc_model_params *params, jfc_model_stats *stats);
 
+/* ---- the widget's own data terms (QGCW:2560-2606 with its "Modeling" and
+ * "Feature" panels, QGCPanel.ui) ------------------------------------------- */
+
+typedef struct {
+  int lazy;              /* 1: lazy snapping (radioButtonLazySnapping), 0: BK */
+  float weight[4];       /* feature channel weights (doubleSpinBoxWeightX..W),
+                            normalised to unit length inside (glm::normalize) */
+  float lazy_neighbor;   /* lambda of graphcut_compute_capacity_lazy */
+  float lazy_terminal;   /* lambda of graphcut_compute_excessflow_lazy */
+  float bk_neighbor;     /* sigma of graphcut_compute_capacity_bk */
+  float bk_terminal;     /* lambda of graphcut_compute_excessflow_bk */
+  int k;                 /* k-means clusters per side, 2..32 (spinBoxK) */
+  int kmeans_iterations; /* 1..256 (spinBoxKMeansIterations) */
+} jfc_widget_model_params;
+
+typedef struct {
+  int64_t seeds[2];          /* seeded nodes: [0] foreground, [1] background */
+  int iterations[2];         /* k-means iterations run per side (lazy) */
+  float centroids[2][32][4]; /* centroids the lazy unary used (weighted features) */
+  uint32_t hist_sum[2];      /* seeded nodes counted by the BK histograms */
+} jfc_widget_model_stats;
+
+/* The panel's defaults (QGCPanel.ui): lazy, weights (0.2126, 0.7152, 0.0722, 0),
+ * lambdas 1 / 1, sigma 0.002, BK terminal 0, k 8, 32 iterations. */
+void jfc_default_widget_model_params(jfc_widget_model_params *p);
+
+/* The "Model" button (slotModelClicked QGCW:2560-2606) on the device, for a
+ * 4-connected 2D or 6-connected 3D instance.  rgba: N x 4 bytes, the
+ * widget's CL_UNORM_INT8 feature image; fg, bg: N bytes, nonzero = seeded
+ * (cl_cut bytes 0 and 1).
+ * - pairwise: graphcut_compute_capacity_lazy / _bk (graphcut_compute_capacity.cl:21-254);
+ * - lazy unary: k-means per side from the seeded levels (QGCW:1997-2107,
+ *   graphcut_histogram.cl:41-60, graphcut_kmeans.cl:21-159), then
+ *   graphcut_compute_excessflow_lazy (graphcut_compute_excessflow.cl:47-77);
+ * - BK unary: 256-bin seeded histograms (graphcut_histogram.cl:19-39, QGCW:3168-3180),
+ *   then graphcut_compute_excessflow_bk (graphcut_compute_excessflow.cl:19-45).
+ * Seeds get ef = +-cl_max_flow (32767); ef > 0 becomes the source capacity and
+ * ef < 0 the sink capacity of the node.  Arithmetic is float32 with every
+ * operation IEEE-rounded in the reference's order (exp and log in double,
+ * rounded once to float); a side with no seeds returns JFC_ERR_NO_SEEDS.
+ * Where the reference is undefined or racy the library fixes one reading:
+ * the BK unary clamps its histogram level to 0..255, non-finite unaries
+ * become 0 (NaN) or +-2^30, k-means takes the highest seeded node index per
+ * level as that level's representative, and centroid sums are exact before
+ * the division. */
+int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, const uint8_t *fg,
+                     const uint8_t *bg, int on_device, const jfc_widget_model_params *params,
+                     jfc_widget_model_stats *stats);
+
 /* ---- graph files (host only; no device work) ----------------------------- */
 
 /* MFI max-flow instance, the format of the reference's CPU benchmarks

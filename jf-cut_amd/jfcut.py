@@ -69,6 +69,18 @@ class ModelStats(ctypes.Structure):
                 ("pair_sum", ctypes.c_int64 * 2), ("pair_count", ctypes.c_int64 * 2)]
 
 
+class WidgetModelParams(ctypes.Structure):
+    _fields_ = [("lazy", ctypes.c_int), ("weight", ctypes.c_float * 4),
+                ("lazy_neighbor", ctypes.c_float), ("lazy_terminal", ctypes.c_float),
+                ("bk_neighbor", ctypes.c_float), ("bk_terminal", ctypes.c_float),
+                ("k", ctypes.c_int), ("kmeans_iterations", ctypes.c_int)]
+
+
+class WidgetModelStats(ctypes.Structure):
+    _fields_ = [("seeds", ctypes.c_int64 * 2), ("iterations", ctypes.c_int * 2),
+                ("centroids", ctypes.c_float * 4 * 32 * 2), ("hist_sum", ctypes.c_uint32 * 2)]
+
+
 class JFCError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libjfcut error {code}: {msg}")
@@ -86,7 +98,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
-           "jfc_autotune", "jfc_get_tile_shape"]
+           "jfc_autotune", "jfc_get_tile_shape", "jfc_default_widget_model_params",
+           "jfc_model_widget"]
 
 
 def lib():
@@ -142,6 +155,9 @@ def lib():
         "jfc_get_tags": (I, [P, I, P, P, I]),
         "jfc_autotune": (I, [P, ctypes.POINTER(Options), I, ctypes.POINTER(I)]),
         "jfc_get_tile_shape": (I, [P, ctypes.POINTER(I)]),
+        "jfc_default_widget_model_params": (None, [ctypes.POINTER(WidgetModelParams)]),
+        "jfc_model_widget": (I, [P, I, P, P, P, I, ctypes.POINTER(WidgetModelParams),
+                                 ctypes.POINTER(WidgetModelStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -184,6 +200,25 @@ def default_options(**kw):
             v = TILE[v]
         setattr(o, k, v)
     return o
+
+
+def widget_model_params(**kw):
+    """jfc_widget_model_params with the panel's defaults, fields overridden by
+    keyword (lazy, weight, lazy_neighbor, lazy_terminal, bk_neighbor,
+    bk_terminal, k, kmeans_iterations)."""
+    p = WidgetModelParams()
+    lib().jfc_default_widget_model_params(ctypes.byref(p))
+    for name, val in kw.items():
+        if name == "weight":
+            for c in range(4):
+                p.weight[c] = float(val[c])
+        elif name == "lazy":
+            p.lazy = int(bool(val))
+        else:
+            if not hasattr(p, name):
+                raise TypeError(f"unknown widget model parameter {name}")
+            setattr(p, name, val)
+    return p
 
 
 class GridGraph:
@@ -263,6 +298,22 @@ class GridGraph:
         st = ModelStats()
         _check(lib().jfc_model_image(self._h, inst, p1, channels, bpc, p2, d1,
                                      ctypes.byref(prm), ctypes.byref(st)))
+        return st
+
+    def model_widget(self, rgba, fg, bg, inst=0, **kw):
+        """The widget's own data terms (jfc_model_widget): rgba is N x 4 uint8
+        (the feature image), fg / bg N bytes, nonzero = seeded.  Keyword
+        arguments override jfc_widget_model_params fields (weight as 4
+        floats).  Returns the stats structure."""
+        prm = widget_model_params(**kw)
+        p1, d1, k1 = _host_or_dev(rgba, np.uint8)
+        p2, d2, k2 = _host_or_dev(fg, np.uint8)
+        p3, d3, k3 = _host_or_dev(bg, np.uint8)
+        if len({d1, d2, d3}) != 1:
+            raise ValueError("rgba, fg and bg must all be host arrays or all device tensors")
+        st = WidgetModelStats()
+        _check(lib().jfc_model_widget(self._h, inst, p1, p2, p3, d1, ctypes.byref(prm),
+                                      ctypes.byref(st)))
         return st
 
     # ---- solve

@@ -11,7 +11,13 @@ where the JF-Cut reference tree is mounted at /root/reference).
 2. ref_flower_cut.npz: the remaining channels of flower's .cut file
    (foreground seeds byte 0, background seeds byte 1, tag byte 3), the output
    of the reference's tag post-process (graphcut_compute_tag.cl) for that cut.
-3. c1_goldens.json: config-1 instances (512x512, 4-connected) built from the
+3. ref_<name>_feature.npz: the widget's feature image of each reference fixture
+   (QGCWidget::initArguments, QGCWidget.cpp:1309-1318: the grey byte
+   (uchar)(intensity * 255) of normalized_<name>.raw in all four channels),
+   the foreground / background seeds of its .cut file, and the widget's
+   Modeling and Feature panel settings stored beside it
+   (Modeling.settings, Feature.settings): inputs of the "Model" button.
+4. c1_goldens.json: config-1 instances (512x512, 4-connected) built from the
    synthetic generator of libjfcut plus the oracle's integer data term, solved by
    BK303: flow, |S|, sha256 of labels and of the capacity planes.
 
@@ -81,6 +87,34 @@ def flower_cut_channels():
                                   (("fg", 0), ("bg", 1), ("tag", 3))})
 
 
+def settings(path):
+    import xml.etree.ElementTree as ET
+    return {el.tag: el.text for el in ET.parse(path).getroot()}
+
+
+def widget_features():
+    for name in PUBLISHED:
+        pre = f"{REF}/{name}/normalized_{name}.raw"
+        nx, ny, nz = resolution(pre + ".dat")
+        vol = np.fromfile(pre, dtype=np.float32)
+        assert vol.size == nx * ny * nz
+        grey = (vol * np.float32(255)).astype(np.uint8)   # float -> uchar truncates
+        cut = np.fromfile(pre + ".cut", dtype=np.uint8).reshape(-1, 4)
+        m = settings(f"{REF}/{name}/Modeling.settings")
+        f = settings(f"{REF}/{name}/Feature.settings")
+        np.savez_compressed(
+            os.path.join(HERE, f"ref_{name}_feature.npz"), nx=np.int64(nx), ny=np.int64(ny),
+            grey=grey, fg=np.packbits(cut[:, 0] != 0), bg=np.packbits(cut[:, 1] != 0),
+            n=np.int64(grey.size), lazy=np.int64(int(m["radioButtonLazySnapping"])),
+            k=np.int64(int(m["spinBoxK"])), iterations=np.int64(int(m["spinBoxKMeansIterations"])),
+            lazy_terminal=np.float64(m["doubleSpinBoxLazyTerminal"]),
+            lazy_neighbor=np.float64(m["doubleSpinBoxLazyNeighbor"]),
+            bk_terminal=np.float64(m["doubleSpinBoxBKTerminal"]),
+            bk_neighbor=np.float64(m["doubleSpinBoxBKNeighbor"]),
+            weight=np.array([float(f["doubleSpinBoxWeight" + c]) for c in "XYZW"]))
+        print(name, "feature", nx, ny, int((cut[:, 0] != 0).sum()), int((cut[:, 1] != 0).sum()))
+
+
 def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
@@ -107,10 +141,12 @@ def c1_goldens(seeds=(0, 1, 2, 3)):
 
 
 if __name__ == "__main__":
-    parts = sys.argv[1:] or ["ref", "flower_cut", "c1"]
+    parts = sys.argv[1:] or ["ref", "flower_cut", "features", "c1"]
     if "ref" in parts:
         ref_fixtures()
     if "flower_cut" in parts:
         flower_cut_channels()
+    if "features" in parts:
+        widget_features()
     if "c1" in parts:
         c1_goldens()
