@@ -81,6 +81,11 @@ typedef struct {
                              push per colour class) as one captured hipGraph
                              (default); 0 = launch them one by one.  profile = 1
                              always launches them one by one. */
+  int tile_order;         /* tile distance field that orders relabel and cut
+                             passes: 0 = separable taxicab distance transform
+                             (default, D launches), 1 = jump flooding
+                             (graphcut_bfs_block.cl:43-63, 2 ceil(log2) + 2
+                             launches); the distances are identical */
 } jfc_options;
 
 #define JFC_TILE_DEFAULT 0

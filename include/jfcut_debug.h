@@ -27,9 +27,10 @@ int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims 
  * host): tile_dist[t] = taxicab distance in tiles to the nearest seed tile of
  * the same instance (or the last bin if the instance has none); order = all
  * tiles sorted by that distance (ties in any order); *max_dist = largest
- * finite distance.  Host outputs of ntiles entries. */
-int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int32_t *tile_dist,
-                         int32_t *order, int32_t *max_dist);
+ * finite distance.  Host outputs of ntiles entries.  method: the
+ * jfc_options.tile_order value (0 distance transform, 1 jump flooding). */
+int jfc_debug_tile_order(jfc_graph *g, int method, const uint8_t *tile_seed,
+                         int32_t *tile_dist, int32_t *order, int32_t *max_dist);
 
 /* Per-tile flags computed by the device from the current input capacities
  * (folded e = cs - ct): which = 0 -> tiles holding a deficit node (relabel

@@ -132,6 +132,8 @@ struct jfc_graph {
   long long *d_red = nullptr, *d_total = nullptr;
   long long *d_def = nullptr;  // per instance: remaining deficit after the cut
   int *d_row_off = nullptr;    // per instance: first row; [count] = rows
+  int *d_lines = nullptr;      // per axis: prefix count of tile lines per instance
+  int lines_total[3] = {0, 0, 0};
   int nrows = 0;
   int *d_sweep = nullptr;
   int sweep_cap = 0;
@@ -170,6 +172,7 @@ struct jfc_graph {
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
+    dfree(d_lines);
   }
   void release() {
     release_tiles();
@@ -189,6 +192,11 @@ struct jfc_graph {
     return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows};
   }
   Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot, rmask}; }
+  TileLines lines() const {
+    const int n1 = (int)inst.size() + 1;
+    return TileLines{{d_lines, d_lines + n1, d_lines + 2 * n1},
+                     {lines_total[0], lines_total[1], lines_total[2]}};
+  }
   long long inst_nodes(int i) const {
     return (long long)inst[i].nx * inst[i].ny * inst[i].nz;
   }
@@ -238,6 +246,20 @@ struct jfc_graph {
             cls[c].push_back(t);
           }
     }
+    // tile lines along x, y, z per instance (separable distance transform)
+    const int n1 = (int)inst.size() + 1;
+    std::vector<int> lines(3 * n1, 0);
+    for (size_t i = 0; i < inst.size(); ++i) {
+      const InstDesc &I = inst[i];
+      const long long per[3] = {(long long)I.ty * I.tz, (long long)I.tx * I.tz,
+                                (long long)I.tx * I.ty};
+      for (int a = 0; a < 3; ++a) {
+        const long long nx = lines[a * n1 + i] + per[a];
+        if (nx >= (1LL << 31) / 4) return fail(JFC_ERR_INVALID, "too many tile lines");
+        lines[a * n1 + i + 1] = (int)nx;
+      }
+    }
+    for (int a = 0; a < 3; ++a) lines_total[a] = lines[a * n1 + n1 - 1];
     std::vector<int> flat;
     color_off.assign(ncolors + 1, 0);
     for (int c = 0; c < ncolors; ++c) {
@@ -259,8 +281,10 @@ struct jfc_graph {
         (rc = dalloc(&d_tdist, ntiles, "tile distance")) ||
         (rc = dalloc(&d_hist, nbins, "tile histogram")) ||
         (rc = dalloc(&d_order, ntiles, "tile order")) ||
-        (rc = dalloc(&d_mat, ntiles, "tile states")))
+        (rc = dalloc(&d_mat, ntiles, "tile states")) ||
+        (rc = dalloc(&d_lines, 3 * n1, "tile lines")))
       return rc;
+    JFC_HIP(hipMemcpy(d_lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_tile_inst, tile_inst.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_color_tiles, flat.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
@@ -310,6 +334,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->initial_relabel = 0;
   o->stall_ratio = 0.99;
   o->graph_sweeps = 1;
+  o->tile_order = 0;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -623,7 +648,8 @@ struct Solver {
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
     launch_tile_order(g->shape, T, g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist, g->d_hist,
-                      g->nbins, g->d_order, g->max_tile_dim, nullptr, st);
+                      g->nbins, g->d_order, g->max_tile_dim, nullptr, g->lines(),
+                      o.tile_order, st);
     launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
@@ -902,6 +928,8 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.max_sweeps > (1 << 24)) S.o.max_sweeps = 1 << 24;
   if (S.o.check_every <= 0) S.o.check_every = 1;
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = 1;
+  if (S.o.tile_order != 0 && S.o.tile_order != 1)
+    return fail(JFC_ERR_INVALID, "tile_order must be 0 (distance transform) or 1 (jump flooding)");
   const int sh = shape_from_option(
       S.o.tile_shape == JFC_TILE_DEFAULT && g->tuned ? g->tuned : S.o.tile_shape, g->dim);
   if (!sh) return fail(JFC_ERR_INVALID, "tile shape does not match the graph's dimensionality");
@@ -1038,12 +1066,14 @@ extern "C" int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t 
   return JFC_OK;
 }
 
-extern "C" int jfc_debug_tile_order(jfc_graph *g, const uint8_t *tile_seed, int32_t *tile_dist,
-                                    int32_t *order, int32_t *max_dist) {
+extern "C" int jfc_debug_tile_order(jfc_graph *g, int method, const uint8_t *tile_seed,
+                                    int32_t *tile_dist, int32_t *order, int32_t *max_dist) {
   if (!g || !tile_seed) return fail(JFC_ERR_INVALID, "null argument");
+  if (method != 0 && method != 1) return fail(JFC_ERR_INVALID, "method must be 0 or 1");
   JFC_HIP(hipMemcpy(g->d_seed, tile_seed, g->ntiles, hipMemcpyHostToDevice));
   launch_tile_order(g->shape, g->tiles(), g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist,
-                    g->d_hist, g->nbins, g->d_order, g->max_tile_dim, g->d_flagword + 2, 0);
+                    g->d_hist, g->nbins, g->d_order, g->max_tile_dim, g->d_flagword + 2,
+                    g->lines(), method, 0);
   JFC_HIP(hipDeviceSynchronize());
   if (tile_dist) JFC_HIP(hipMemcpy(tile_dist, g->d_tdist, g->ntiles * 4, hipMemcpyDeviceToHost));
   if (order) JFC_HIP(hipMemcpy(order, g->d_order, g->ntiles * 4, hipMemcpyDeviceToHost));

@@ -99,10 +99,16 @@ void launch_cut_init(int shape, const Tiles &T, const Planes &P,
                      unsigned char *seed, long long *def, hipStream_t st);
 
 // Jump-flooding tile distance + counting sort of tiles by distance.
+// Lines of the tile grids along each axis: prefix[a][i] = lines along axis a
+// in the instances before i, total[a] = all of them.
+struct TileLines {
+  const int *prefix[3];
+  int total[3];
+};
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *jf0, int *jf1, int *tdist, int *hist, int nbins,
                        int *order, int max_tile_dim, int *max_dist,
-                       hipStream_t st);
+                       const TileLines &lines, int jump_flooding, hipStream_t st);
 
 // state = {count of the running sweep, sweep index}: hist[index] = count,
 // then the next sweep starts at 0.

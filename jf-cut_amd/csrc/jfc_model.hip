@@ -134,13 +134,10 @@ __global__ void k_unary(const void *img, int bpc, int C, long long n,
 }
 
 template <bool D3>
-__device__ __forceinline__ bool nb_of(const InstDesc &I, long long v, int d, long long &u) {
+__device__ __forceinline__ bool nb_at(const InstDesc &I, int x, int y, int z, int d,
+                                      long long &u) {
   int dx, dy, dz;
   dir_of<D3>(d, dx, dy, dz);
-  const int x = (int)(v % I.nx);
-  const long long r = v / I.nx;
-  const int y = (int)(r % I.ny);
-  const int z = (int)(r / I.ny);
   const int X = x + dx, Y = y + dy, Z = z + dz;
   if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) return false;
   u = X + (long long)I.nx * (Y + (long long)I.ny * Z);
@@ -164,16 +161,20 @@ __global__ __launch_bounds__(256) void k_pair_sums(InstDesc I, const void *img, 
                                                    int C, int zsep, long long *sums) {
   __shared__ long long buf[4][4];
   long long s0 = 0, p0 = 0, s1 = 0, p1 = 0;
-  const long long n = (long long)I.nx * I.ny * I.nz;
-  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
-       v += (long long)gridDim.x * blockDim.x)
-    for (int d = 0; d < I.K; d += 2) {
-      long long u;
-      if (!nb_of<D3>(I, v, d, u)) continue;
-      const long long D = sqdiff(img, bpc, C, v, u);
-      if (D3 && d == 4 && zsep) { s1 += D; p1 += 1; }
-      else { s0 += D; p0 += 1; }
+  const long long rows = (long long)I.ny * I.nz;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int y = (int)(row % I.ny), z = (int)(row / I.ny);
+    for (int x = threadIdx.x; x < I.nx; x += blockDim.x) {
+      const long long v = x + (long long)I.nx * row;
+      for (int d = 0; d < I.K; d += 2) {
+        long long u;
+        if (!nb_at<D3>(I, x, y, z, d, u)) continue;
+        const long long D = sqdiff(img, bpc, C, v, u);
+        if (D3 && d == 4 && zsep) { s1 += D; p1 += 1; }
+        else { s0 += D; p0 += 1; }
+      }
     }
+  }
   long long vals[4] = {s0, p0, s1, p1};
   const int w = threadIdx.x >> 6;
   for (int j = 0; j < 4; ++j) {
@@ -189,7 +190,9 @@ __global__ __launch_bounds__(256) void k_pair_sums(InstDesc I, const void *img, 
   }
 }
 
-// w(D) = #{k : D <= T[k]} with tables 0: x/y, 1: 2D diagonals, 2: z.
+// w(D) = #{k : D <= T[k]} with tables 0: x/y, 1: 2D diagonals, 2: z.  Every
+// table is non-increasing (thresholds() below), so the count is the length
+// of the prefix with D <= T[k], found by binary search.
 template <bool D3>
 __global__ __launch_bounds__(256) void k_pairwise(InstDesc I, const void *img, int bpc,
                                                   int C, const long long *T,
@@ -200,21 +203,29 @@ __global__ __launch_bounds__(256) void k_pairwise(InstDesc I, const void *img, i
   for (int i = threadIdx.x; i < 3 * TMAX; i += blockDim.x) s_T[i] = T[i];
   if (threadIdx.x < 3) s_n[threadIdx.x] = nT[threadIdx.x];
   __syncthreads();
-  const long long n = (long long)I.nx * I.ny * I.nz;
-  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
-       v += (long long)gridDim.x * blockDim.x)
-    for (int d = 0; d < I.K; ++d) {
-      long long u;
-      int w = 0;
-      if (nb_of<D3>(I, v, d, u)) {
-        const long long D = sqdiff(img, bpc, C, v, u);
-        const int tb = D3 ? (d >= 4 ? 2 : 0) : (d >= 4 ? 1 : 0);
-        const long long *t = s_T + tb * TMAX;
-        const int m = s_n[tb];
-        while (w < m && D <= t[w]) ++w;
+  const long long rows = (long long)I.ny * I.nz;
+  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int y = (int)(row % I.ny), z = (int)(row / I.ny);
+    for (int x = threadIdx.x; x < I.nx; x += blockDim.x) {
+      const long long v = x + (long long)I.nx * row;
+      for (int d = 0; d < I.K; ++d) {
+        long long u;
+        int w = 0;
+        if (nb_at<D3>(I, x, y, z, d, u)) {
+          const long long D = sqdiff(img, bpc, C, v, u);
+          const int tb = D3 ? (d >= 4 ? 2 : 0) : (d >= 4 ? 1 : 0);
+          const long long *t = s_T + tb * TMAX;
+          int hi = s_n[tb];
+          while (w < hi) {
+            const int mid = (w + hi) >> 1;
+            if (D <= t[mid]) w = mid + 1;
+            else hi = mid;
+          }
+        }
+        c0[d * stride + I.node_off + v] = w;
       }
-      c0[d * stride + I.node_off + v] = w;
     }
+  }
 }
 
 int thresholds(double lambda, long long S, long long P, long long *T) {
@@ -371,8 +382,9 @@ extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int ch
                                prm->unary_scale, cs + I.node_off, ct + I.node_off);
     // pairwise terms
     MODEL_HIP(hipMemset(d_sums, 0, 4 * sizeof(long long)));
-    if (d3) k_pair_sums<true><<<grid1(n), 256>>>(I, img, bpc, C, prm->z_separate, d_sums);
-    else k_pair_sums<false><<<grid1(n), 256>>>(I, img, bpc, C, 0, d_sums);
+    const int rgrid = (int)std::min<long long>((long long)I.ny * I.nz, 65536);
+    if (d3) k_pair_sums<true><<<rgrid, 256>>>(I, img, bpc, C, prm->z_separate, d_sums);
+    else k_pair_sums<false><<<rgrid, 256>>>(I, img, bpc, C, 0, d_sums);
     MODEL_HIP(hipMemcpy(sums, d_sums, sizeof(sums), hipMemcpyDeviceToHost));
     ST.pair_sum[0] = sums[0]; ST.pair_count[0] = sums[1];
     ST.pair_sum[1] = sums[2]; ST.pair_count[1] = sums[3];
@@ -382,8 +394,8 @@ extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int ch
     else nT[2] = thresholds(prm->lambda_z, sums[0], sums[1], Tab.data() + 2 * TMAX);
     MODEL_HIP(hipMemcpy(d_T, Tab.data(), Tab.size() * sizeof(long long), hipMemcpyHostToDevice));
     MODEL_HIP(hipMemcpy(d_nT, nT, sizeof(nT), hipMemcpyHostToDevice));
-    if (d3) k_pairwise<true><<<grid1(n), 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
-    else k_pairwise<false><<<grid1(n), 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
+    if (d3) k_pairwise<true><<<rgrid, 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
+    else k_pairwise<false><<<rgrid, 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
     MODEL_HIP(hipDeviceSynchronize());
     graph_mark_inputs(g, inst);
   }

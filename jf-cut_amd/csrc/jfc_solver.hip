@@ -730,6 +730,94 @@ __global__ void k_jf_dist(Tiles T, const int *jf, int *tdist, int nbins) {
   tdist[t] = d;
 }
 
+// ---- the same distances by a separable taxicab distance transform ---------------
+//
+// The taxicab distance to the nearest seed tile separates by axis:
+// d(p) = min over lines of the 1D transform g(p) = min_q (f(q) + |p - q|),
+// applied along x, then y (then z).  On one line, g = min(F, B) with the
+// prefix minimum F(p) = p + min_{q<=p} (f(q) - q) and the suffix minimum
+// B(p) = -p + min_{q>=p} (f(q) + q).  One wavefront owns a line: each lane
+// scans a contiguous chunk, and the chunk minima are combined across lanes
+// by shuffles.  The result equals jump flooding's exactly, in D launches
+// instead of 2 ceil(log2(max tile dim)) + 2.
+constexpr int kDtBig = 1 << 28;   // "no seed yet" (far above any distance)
+
+__device__ __forceinline__ int wave_excl_prefix_min(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v = min(v, u);
+  }
+  const int e = __shfl_up(v, 1);
+  return lane == 0 ? INT_MAX : e;
+}
+
+__device__ __forceinline__ int wave_excl_suffix_min(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_down(v, o);
+    if (lane + o < 64) v = min(v, u);
+  }
+  const int e = __shfl_down(v, 1);
+  return lane == 63 ? INT_MAX : e;
+}
+
+// One axis of the transform over every line of every instance.  `lines`
+// holds, for this axis, the prefix count of lines per instance.  FIRST reads
+// the seed flags (0 / kDtBig); LAST writes the clamped distance bins of
+// k_jf_dist (nbins - 1 for instances without a seed tile).  `out` also holds
+// the prefix pass before the suffix pass combines with it.
+template <int AXIS, bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_dt_line(Tiles T, const int *lines, int nlines,
+                                                 const unsigned char *seed, const int *in,
+                                                 int *out, int *tdist, int nbins) {
+  const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (line >= nlines) return;   // whole wave
+  int lo = 0, hi = T.ninst - 1;  // instance: last with lines[i] <= line
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (lines[mid] <= line) lo = mid;
+    else hi = mid - 1;
+  }
+  const InstDesc I = T.inst[lo];
+  const int l = line - lines[lo];
+  int L, step, base;
+  if (AXIS == 0) {
+    L = I.tx; step = 1; base = I.tx * l;                                  // l = y + ty z
+  } else if (AXIS == 1) {
+    L = I.ty; step = I.tx; base = (l % I.tx) + I.tx * I.ty * (l / I.tx);  // l = x + tx z
+  } else {
+    L = I.tz; step = I.tx * I.ty; base = l;                               // l = x + tx y
+  }
+  base += I.tile_off;
+  const int lane = lane_id();
+  const int c = (L + 63) >> 6;
+  const int p0 = min(L, lane * c), p1 = min(L, p0 + c);
+  auto val = [&](int p) -> int {
+    const int t = base + p * step;
+    return FIRST ? (seed[t] ? 0 : kDtBig) : in[t];
+  };
+  int m = INT_MAX;
+  for (int p = p0; p < p1; ++p) m = min(m, val(p) - p);
+  int run = wave_excl_prefix_min(m);
+  for (int p = p0; p < p1; ++p) {
+    run = min(run, val(p) - p);
+    out[base + p * step] = run + p;
+  }
+  m = INT_MAX;
+  for (int p = p0; p < p1; ++p) m = min(m, val(p) + p);
+  run = wave_excl_suffix_min(m);
+  for (int p = p1 - 1; p >= p0; --p) {
+    run = min(run, val(p) + p);
+    const int t = base + p * step;
+    const int d = min(out[t], run - p);
+    if (LAST) tdist[t] = d >= kDtBig / 2 ? nbins - 1 : min(d, nbins - 2);
+    else out[t] = d;
+  }
+}
+
 // Largest finite tile distance (reduce_histogram, reduce.cl:56-141).
 __global__ void k_max_dist(const int *tdist, int n, int unreachable, int *out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -807,20 +895,35 @@ __global__ __launch_bounds__(256) void k_scatter_order(const int *tdist, int n, 
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *jf0, int *jf1, int *tdist, int *hist, int nbins,
                        int *order, int max_tile_dim, int *max_dist,
-                       hipStream_t st) {
+                       const TileLines &lines, int jump_flooding, hipStream_t st) {
   const int n = T.ntiles;
   const int g = (n + 255) / 256;
   const int d3 = shape_is_3d(shape) ? 1 : 0;
-  k_jf_init<<<g, 256, 0, st>>>(T, seed, jf0);
-  int *a = jf0, *b = jf1;
-  int top = 1;
-  while (top < max_tile_dim) top <<= 1;
-  for (int step = top >> 1; step >= 1; step >>= 1)
-    for (int rep = 0; rep < 2; ++rep) {
-      k_jf_pass<<<g, 256, 0, st>>>(T, a, b, step, d3);
-      int *tmp = a; a = b; b = tmp;
+  if (jump_flooding) {
+    k_jf_init<<<g, 256, 0, st>>>(T, seed, jf0);
+    int *a = jf0, *b = jf1;
+    int top = 1;
+    while (top < max_tile_dim) top <<= 1;
+    for (int step = top >> 1; step >= 1; step >>= 1)
+      for (int rep = 0; rep < 2; ++rep) {
+        k_jf_pass<<<g, 256, 0, st>>>(T, a, b, step, d3);
+        int *tmp = a; a = b; b = tmp;
+      }
+    k_jf_dist<<<g, 256, 0, st>>>(T, a, tdist, nbins);
+  } else {
+    auto gl = [](int nl) { return (nl + 3) / 4; };
+    k_dt_line<0, true, false><<<gl(lines.total[0]), 256, 0, st>>>(
+        T, lines.prefix[0], lines.total[0], seed, nullptr, jf0, nullptr, nbins);
+    if (d3) {
+      k_dt_line<1, false, false><<<gl(lines.total[1]), 256, 0, st>>>(
+          T, lines.prefix[1], lines.total[1], nullptr, jf0, jf1, nullptr, nbins);
+      k_dt_line<2, false, true><<<gl(lines.total[2]), 256, 0, st>>>(
+          T, lines.prefix[2], lines.total[2], nullptr, jf1, jf0, tdist, nbins);
+    } else {
+      k_dt_line<1, false, true><<<gl(lines.total[1]), 256, 0, st>>>(
+          T, lines.prefix[1], lines.total[1], nullptr, jf0, jf1, tdist, nbins);
     }
-  k_jf_dist<<<g, 256, 0, st>>>(T, a, tdist, nbins);
+  }
   if (max_dist) {
     launch_fill_i32(max_dist, 1, 0, st);
     k_max_dist<<<g, 256, 0, st>>>(tdist, n, nbins - 1, max_dist);

@@ -37,7 +37,8 @@ class Options(ctypes.Structure):
                 ("relabel_work", ctypes.c_double), ("max_local_iters", ctypes.c_int),
                 ("tile_shape", ctypes.c_int), ("profile", ctypes.c_int),
                 ("stream", ctypes.c_void_p), ("initial_relabel", ctypes.c_int),
-                ("stall_ratio", ctypes.c_double), ("graph_sweeps", ctypes.c_int)]
+                ("stall_ratio", ctypes.c_double), ("graph_sweeps", ctypes.c_int),
+                ("tile_order", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -136,7 +137,7 @@ def lib():
         "jfc_get_terminal_caps": (I, [P, I, P, P, I]),
         "jfc_get_neighbor_caps": (I, [P, I, I, P, I]),
         "jfc_debug_tiles": (I, [P, I, P, P, P, P]),
-        "jfc_debug_tile_order": (I, [P, P, P, P, P]),
+        "jfc_debug_tile_order": (I, [P, I, P, P, P, P]),
         "jfc_debug_sweep_counts": (I, [P, P, I]),
         "jfc_backward_all": (I, [P, P, P, P, I]),
         "jfc_debug_tile_flags": (I, [P, I, P]),
@@ -420,7 +421,7 @@ class GridGraph:
         _check(lib().jfc_debug_tiles(self._h, inst, ctypes.byref(nt), td, gd, ctypes.byref(off)))
         return nt.value, tuple(td), tuple(gd), off.value
 
-    def debug_tile_order(self, tile_seed):
+    def debug_tile_order(self, tile_seed, method=0):
         nt = self.debug_tiles(0)[0]
         seed = np.ascontiguousarray(tile_seed, dtype=np.uint8)
         assert seed.size == nt
@@ -428,7 +429,7 @@ class GridGraph:
         order = np.zeros(nt, np.int32)
         mx = ctypes.c_int32()
         vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
-        _check(lib().jfc_debug_tile_order(self._h, vp(seed), vp(dist), vp(order),
+        _check(lib().jfc_debug_tile_order(self._h, method, vp(seed), vp(dist), vp(order),
                                           ctypes.byref(mx)))
         return dist, order, mx.value
 

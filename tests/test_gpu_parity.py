@@ -291,11 +291,13 @@ def test_tile_flags_gpu(gpu, shape):
     g.close()
 
 
-def test_tile_order_gpu(gpu):
-    """Jump flooding over the tile grid = brute-force taxicab distance to the
-    nearest seed tile; the order is sorted by it; max distance = reduce."""
+@pytest.mark.parametrize("method", [0, 1])
+def test_tile_order_gpu(gpu, method):
+    """Distance transform (method 0) and jump flooding (method 1) over the
+    tile grid = brute-force taxicab distance to the nearest seed tile; the
+    order is sorted by it; max distance = reduce."""
     rng = np.random.default_rng(8)
-    for shape in [(640, 360, 1, 4), (1000, 70, 1, 8), (96, 80, 60, 6)]:
+    for shape in [(640, 360, 1, 4), (1000, 70, 1, 8), (96, 80, 60, 6), (20000, 9, 1, 4)]:
         g = gpu.GridGraph([shape, shape])
         nt, td, gd, off = g.debug_tiles(0)
         _, _, gd1, off1 = g.debug_tiles(1)
@@ -303,7 +305,7 @@ def test_tile_order_gpu(gpu):
         seeds[off1:] = 0
         seeds[5] = 1
         seeds[off1 + 3] = 1
-        dist, order, mx = g.debug_tile_order(seeds)
+        dist, order, mx = g.debug_tile_order(seeds, method)
         for inst, o in ((0, off), (1, off1)):
             tx, ty, tz = gd
             n = tx * ty * tz
