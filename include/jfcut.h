@@ -86,7 +86,16 @@ typedef struct {
                              (default, D launches), 1 = jump flooding
                              (graphcut_bfs_block.cl:43-63, 2 ceil(log2) + 2
                              launches); the distances are identical */
+  int method;             /* JFC_METHOD_JFCUT (0): tile push-relabel in LDS;
+                             JFC_METHOD_FASTCUT (1): the widget's Fast-Cut
+                             comparator (radioButtonCutMethodJFCut unchecked,
+                             QGCW:2419-2427): Jacobi relabel plus line pushes
+                             along each axis, 4- and 6-connected graphs only.
+                             Same flow and labels either way. */
 } jfc_options;
+
+#define JFC_METHOD_JFCUT 0
+#define JFC_METHOD_FASTCUT 1
 
 #define JFC_TILE_DEFAULT 0
 #define JFC_TILE_32x8 1

@@ -335,6 +335,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->stall_ratio = 0.99;
   o->graph_sweeps = 1;
   o->tile_order = 0;
+  o->method = JFC_METHOD_JFCUT;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -750,6 +751,20 @@ struct Solver {
     launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, s_);
   }
 
+  // One Fast-Cut sweep (QGCWidget::relabelPushFast, QGCW:2156-2205): every
+  // active tile in one list, Jacobi relabel, line pushes, active flags.
+  void issue_fast_sweep() {
+    Queue q = queue_from_list();
+    launch_compact(g->d_color_tiles, g->ntiles, 0, g->d_flag, nullptr, q, g->d_sweep_state,
+                   g->d_total, st);
+    hipEvent_t a = o.profile ? tm.mark() : nullptr;
+    launch_fc_sweep(g->shape, std::min(g->nwg, g->ntiles), g->tiles(), g->planes(), q,
+                    g->d_flag, st);
+    if (o.profile) push_ev.push_back({a, tm.mark()});
+    advance();
+    launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, st);
+  }
+
   // Launch one sweep as a hipGraph captured on first use (per starting
   // slot, stream and local-iteration cap; tile-table or counter changes drop
   // the captured graphs).  The sweep contains no memset node.
@@ -808,6 +823,8 @@ struct Solver {
     JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
     launch_fill_u8(g->d_mat, g->ntiles, 0, st);
+    // Fast-Cut reads and writes the residual planes directly
+    if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
     std::vector<long long> red((size_t)nI * RED_SLOTS);
     JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
@@ -834,12 +851,17 @@ struct Solver {
       while (sweep < o.max_sweeps) {
         // the first sweep of a solve runs eagerly (it also primes the
         // launchers' occupancy queries); later ones replay a captured graph
-        if (o.graph_sweeps && !o.profile && sweep > 0) {
-          if ((rc = replay_sweep())) return rc;
+        if (o.method == JFC_METHOD_FASTCUT) {
+          issue_fast_sweep();
+          s.push_launches += 2 + 2 * g->dim;
         } else {
-          issue_sweep(st, true);
+          if (o.graph_sweeps && !o.profile && sweep > 0) {
+            if ((rc = replay_sweep())) return rc;
+          } else {
+            issue_sweep(st, true);
+          }
+          s.push_launches += g->ncolors;
         }
-        s.push_launches += g->ncolors;
         ++sweep;
         if (++since_check >= o.check_every || sweep == o.max_sweeps) {
           since_check = 0;
@@ -930,6 +952,12 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = 1;
   if (S.o.tile_order != 0 && S.o.tile_order != 1)
     return fail(JFC_ERR_INVALID, "tile_order must be 0 (distance transform) or 1 (jump flooding)");
+  if (S.o.method != JFC_METHOD_JFCUT && S.o.method != JFC_METHOD_FASTCUT)
+    return fail(JFC_ERR_INVALID, "unknown method");
+  if (S.o.method == JFC_METHOD_FASTCUT)
+    for (const InstDesc &I : g->inst)
+      if (I.K == 8)
+        return fail(JFC_ERR_INVALID, "Fast-Cut runs on 4- and 6-connected graphs only");
   const int sh = shape_from_option(
       S.o.tile_shape == JFC_TILE_DEFAULT && g->tuned ? g->tuned : S.o.tile_shape, g->dim);
   if (!sh) return fail(JFC_ERR_INVALID, "tile shape does not match the graph's dimensionality");

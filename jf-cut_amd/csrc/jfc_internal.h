@@ -128,6 +128,10 @@ void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mas
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
                  int max_iters, long long *iter_total, hipStream_t st);
+// Fast-Cut comparator (jfc_fastcut.hip)
+void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);
+void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
+                     unsigned char *flag, hipStream_t st);
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
                   hipStream_t st);

@@ -67,7 +67,8 @@ def test_random_grids_gpu(gpu, conn, shape):
 @pytest.mark.parametrize("opts", [dict(tile_shape="64x4"), dict(tile_shape="16x16"),
                                   dict(max_local_iters=1), dict(relabel_work=0.0),
                                   dict(check_every=1, max_local_iters=1000),
-                                  dict(graph_sweeps=0), dict(check_every=3)])
+                                  dict(graph_sweeps=0), dict(check_every=3), dict(tile_order=1),
+                                  dict(initial_relabel=1)])
 def test_options_do_not_change_result(gpu, opts):
     rng = np.random.default_rng(99)
     nx, ny = 203, 147
@@ -79,6 +80,58 @@ def test_options_do_not_change_result(gpu, opts):
         assert g.flow() == f
         assert np.array_equal(g.labels(), l)
         g.close()
+
+
+def serpentine(nx, ny, nz=1, cap=5, seed=0):
+    """A single corridor snaking through the grid (rows of each z slice,
+    then slices): its residual breadth-first distances cross the tile grid
+    back and forth, the worst case for pass-by-pass relabelling."""
+    rng = np.random.default_rng(seed)
+    N = nx * ny * nz
+    conn = 6 if nz > 1 else 4
+    caps = np.zeros((conn, N), np.int32)
+    idx = lambda x, y, z: x + nx * (y + ny * z)
+    # one slice: even rows left-right-left..., joined through the odd rows
+    snake = []
+    for k, y in enumerate(range(0, ny, 2)):
+        xs = range(nx) if k % 2 == 0 else range(nx - 1, -1, -1)
+        if snake:
+            snake.append((snake[-1][0], y - 1))
+        snake += [(x, y) for x in xs]
+    # slices alternate direction, joined by one step along z
+    path = []
+    for z in range(nz):
+        path += [(x, y, z) for x, y in (snake if z % 2 == 0 else snake[::-1])]
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)][:conn]
+    for (a, b) in zip(path, path[1:]):
+        d = (b[0] - a[0], b[1] - a[1], b[2] - a[2])
+        k = dirs.index(d)
+        caps[k, idx(*a)] = cap + rng.integers(0, 3)
+        caps[k ^ 1, idx(*b)] = cap + rng.integers(0, 3)
+    cs = np.zeros(N, np.int32)
+    ct = np.zeros(N, np.int32)
+    cs[idx(*path[0])] = 1000
+    ct[idx(*path[-1])] = 1000
+    on = np.array([idx(*p) for p in path])
+    cs[on[::7]] += rng.integers(0, 4, on[::7].size).astype(np.int32)
+    ct[on[3::11]] += rng.integers(0, 4, on[3::11].size).astype(np.int32)
+    return cs, ct, caps, conn
+
+
+@pytest.mark.parametrize("shape", [(256, 63, 1), (64, 31, 6)])
+def test_relabel_serpentine(gpu, shape):
+    """Exact relabel and cut along one long winding residual path."""
+    nx, ny, nz = shape
+    cs, ct, caps, conn = serpentine(nx, ny, nz)
+    f, l, ok = ol.maxflow(nx, ny, nz, conn, cs, ct, caps)
+    for opts in (dict(), dict(tile_order=1), dict(max_local_iters=1)):
+        for ir in (0, 1):
+            say("case serpentine", shape, opts, ir)
+            g, st = solve_grid(gpu, nx, ny, nz, conn, cs, ct, caps, initial_relabel=ir, **opts)
+            assert st.converged == 1
+            assert g.flow() == f
+            assert np.array_equal(g.labels(), l)
+            g.close()
 
 
 def test_sweep_graph_replays(gpu):
