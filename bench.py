@@ -163,6 +163,27 @@ def cpu_baseline(wl, sample_w, sample_h):
     }
 
 
+def fastcut_comparator(reps=5):
+    """The paper's in-tree GPU comparison on this GPU: the same 4-connected
+    512x512 cut (config 1) solved by the tile push-relabel (JF-Cut) and by
+    the Fast-Cut line pushes (jfc_options.method = 1); device ms per solve,
+    flows equal."""
+    wl = WORKLOADS["c1"]
+    g = build_graph(wl, seed=0)
+    out = {"workload": wl["name"]}
+    flows = {}
+    for name, m in (("jfcut", 0), ("fastcut", 1)):
+        g.solve(method=m)
+        ms = [g.solve(method=m).ms_total for _ in range(reps)]
+        flows[name] = g.flow()
+        out[name + "_ms"] = round(float(np.median(ms)), 3)
+    g.close()
+    assert flows["jfcut"] == flows["fastcut"], flows
+    out["jfcut_speedup"] = round(out["fastcut_ms"] / out["jfcut_ms"], 2)
+    out["flows_equal"] = True
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +192,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-comparator", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -236,6 +258,11 @@ def main():
         batched = batched_forward_backward()
         log(f"batched fwd+bwd: {batched}")
 
+    comparator = None
+    if not args.no_comparator:
+        comparator = fastcut_comparator()
+        log(f"fast-cut comparator: {comparator}")
+
     cpu = None
     if not args.no_cpu_baseline:
         sw, sh = [int(v) for v in args.cpu_sample.split("x")]
@@ -266,6 +293,7 @@ def main():
                      "avg_launch_ms": round(push_ms / max(1, push_launches), 4)},
         "cpu_baseline": cpu,
         "batched_fwd_bwd": batched,
+        "fastcut_comparator": comparator,
     }
     print(json.dumps(line), flush=True)
 
