@@ -109,7 +109,10 @@ struct jfc_graph {
   std::string size_class;  // key of the process-wide autotuning table
   int Kg = 4;  // planes stored (max K over instances)
   int shape = 0;
-  long long Ntot = 0;
+  long long Ntot = 0;   // nodes stored, rows padded (plane stride)
+  long long Nuser = 0;  // nodes of the instances as callers number them
+  void *stage = nullptr;  // device staging of caller arrays (rows of nx)
+  size_t stage_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<int> caps_set;  // per instance: bit 0 terminals, bit 1 + d neighbour planes
   // tile tables for the current shape
@@ -179,6 +182,9 @@ struct jfc_graph {
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
     dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
     dfree(d_def); dfree(d_row_off); dfree(d_sweep_state);
+    if (stage) (void)hipFree(stage);
+    stage = nullptr;
+    stage_bytes = 0;
     if (priv_event) (void)hipEventDestroy(priv_event);
     if (priv_stream) (void)hipStreamDestroy(priv_stream);
     priv_event = nullptr;
@@ -199,6 +205,9 @@ struct jfc_graph {
   }
   long long inst_nodes(int i) const {
     return (long long)inst[i].nx * inst[i].ny * inst[i].nz;
+  }
+  long long inst_stored(int i) const {
+    return (long long)inst[i].px * inst[i].ny * inst[i].nz;
   }
 
   // Build tile tables, colour classes and instance descriptors for `sh`.
@@ -348,7 +357,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
     return fail(JFC_ERR_DEVICE, "no HIP device available");
   }
   jfc_graph *g = new jfc_graph();
-  long long off = 0;
+  long long off = 0, uoff = 0;
   g->dim = shapes[2] > 1 ? 3 : 2;
   g->Kg = 0;
   for (int i = 0; i < count; ++i) {
@@ -366,13 +375,17 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
     }
     InstDesc I{};
     I.nx = nx; I.ny = ny; I.nz = nz; I.conn = conn; I.K = conn;
+    I.px = (int)(((long long)nx + 31) / 32 * 32);
     I.node_off = off;
-    off += (long long)nx * ny * nz;
+    I.user_off = uoff;
+    off += (long long)I.px * ny * nz;
+    uoff += (long long)nx * ny * nz;
     g->inst.push_back(I);
     g->Kg = std::max(g->Kg, conn);
   }
   if (off >= (1LL << 40)) { delete g; return fail(JFC_ERR_INVALID, "graph too large"); }
   g->Ntot = off;
+  g->Nuser = uoff;
   g->caps_set.assign(count, 0);
   g->flows.assign(count, 0);
   int rc;
@@ -410,9 +423,15 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       return fail(JFC_ERR_DEVICE, "row table upload failed");
     }
   }
+  // everything zero, so the pad nodes of the rows stay isolated and empty
   launch_fill_i32(g->cs, off, 0, 0);
   launch_fill_i32(g->ct, off, 0, 0);
   launch_fill_i32(g->c0, off * g->Kg, 0, 0);
+  launch_fill_i32(g->cap, off * g->Kg, 0, 0);
+  launch_fill_i32(g->e, off, 0, 0);
+  launch_fill_i32(g->h, off, 0, 0);
+  launch_fill_i32((int *)g->mask0, off, 0, 0);
+  launch_fill_i32((int *)g->rmask, off, 0, 0);
   launch_fill_u8(g->lab, off, 0, 0);
   int dev = 0;
   hipDeviceProp_t prop;
@@ -421,7 +440,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
   {
     unsigned conns = 0;
     for (const auto &I : g->inst) conns |= 1u << I.conn;
-    const double mean = (double)off / count;
+    const double mean = (double)uoff / count;
     g->size_class = std::to_string(g->dim) + ":" + std::to_string(conns) + ":" +
                     std::to_string(log2_round(count)) + ":" + std::to_string(log2_round(mean));
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -469,10 +488,62 @@ static int check_inst(jfc_graph *g, int inst) {
   return JFC_OK;
 }
 
-static int put_plane(int *dst, const int32_t *src, long long n, int on_device) {
-  JFC_HIP(hipMemcpy(dst, src, n * sizeof(int32_t),
-                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+// Caller arrays (rows of nx) <-> stored planes (rows of px); pad entries of
+// the stored side are left as they are (zero from creation).  Host arrays
+// cross PCIe in one contiguous copy through the graph's device staging
+// buffer; the row scatter / gather is a device-to-device 2D copy.
+static int stage_for(jfc_graph *g, size_t bytes, void **p) {
+  if (g->stage_bytes < bytes) {
+    if (g->stage) (void)hipFree(g->stage);
+    g->stage = nullptr;
+    g->stage_bytes = 0;
+    if (hipMalloc(&g->stage, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(JFC_ERR_OOM, "out of device memory: staging buffer");
+    }
+    g->stage_bytes = bytes;
+  }
+  *p = g->stage;
   return JFC_OK;
+}
+static int put_rows(jfc_graph *g, const InstDesc &I, void *dst, const void *src, size_t elem,
+                    int on_device) {
+  const size_t row = (size_t)I.nx * elem, rows = (size_t)I.ny * I.nz;
+  if (I.px == I.nx) {
+    JFC_HIP(hipMemcpy(dst, src, row * rows,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    return JFC_OK;
+  }
+  if (!on_device) {
+    void *tmp = nullptr;
+    if (int rc = stage_for(g, row * rows, &tmp)) return rc;
+    JFC_HIP(hipMemcpy(tmp, src, row * rows, hipMemcpyHostToDevice));
+    src = tmp;
+  }
+  JFC_HIP(hipMemcpy2D(dst, (size_t)I.px * elem, src, row, row, rows, hipMemcpyDeviceToDevice));
+  return JFC_OK;
+}
+static int get_rows(jfc_graph *g, const InstDesc &I, void *dst, const void *src, size_t elem,
+                    int on_device) {
+  const size_t row = (size_t)I.nx * elem, rows = (size_t)I.ny * I.nz;
+  if (I.px == I.nx) {
+    JFC_HIP(hipMemcpy(dst, src, row * rows,
+                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    return JFC_OK;
+  }
+  if (on_device) {
+    JFC_HIP(hipMemcpy2D(dst, row, src, (size_t)I.px * elem, row, rows, hipMemcpyDeviceToDevice));
+    return JFC_OK;
+  }
+  void *tmp = nullptr;
+  if (int rc = stage_for(g, row * rows, &tmp)) return rc;
+  JFC_HIP(hipMemcpy2D(tmp, row, src, (size_t)I.px * elem, row, rows, hipMemcpyDeviceToDevice));
+  JFC_HIP(hipMemcpy(dst, tmp, row * rows, hipMemcpyDeviceToHost));
+  return JFC_OK;
+}
+static int put_plane(jfc_graph *g, const InstDesc &I, int *dst, const int32_t *src,
+                     int on_device) {
+  return put_rows(g, I, dst, src, sizeof(int32_t), on_device);
 }
 
 static int validate(jfc_graph *g, const int *p, long long n, int lo, int hi,
@@ -492,10 +563,10 @@ extern "C" int jfc_set_terminal_caps(jfc_graph *g, int inst, const int32_t *cap_
   int rc;
   if ((rc = check_inst(g, inst))) return rc;
   if (!cap_source || !cap_sink) return fail(JFC_ERR_INVALID, "null capacity pointer");
-  const long long n = g->inst_nodes(inst);
+  const long long n = g->inst_stored(inst);
   const long long o = g->inst[inst].node_off;
-  if ((rc = put_plane(g->cs + o, cap_source, n, on_device)) ||
-      (rc = put_plane(g->ct + o, cap_sink, n, on_device)))
+  if ((rc = put_plane(g, g->inst[inst], g->cs + o, cap_source, on_device)) ||
+      (rc = put_plane(g, g->inst[inst], g->ct + o, cap_sink, on_device)))
     return rc;
   if ((rc = validate(g, g->cs + o, n, 0, JFC_MAX_TERMINAL_CAP, "source capacity")) ||
       (rc = validate(g, g->ct + o, n, 0, JFC_MAX_TERMINAL_CAP, "sink capacity"))) {
@@ -513,11 +584,12 @@ extern "C" int jfc_set_excess(jfc_graph *g, int inst, const int32_t *ef, int on_
   int rc;
   if ((rc = check_inst(g, inst))) return rc;
   if (!ef) return fail(JFC_ERR_INVALID, "null excess pointer");
-  const long long n = g->inst_nodes(inst);
+  const long long n = g->inst_stored(inst);
   const long long o = g->inst[inst].node_off;
   int *tmp = nullptr;
   if ((rc = dalloc(&tmp, n, "excess staging"))) return rc;
-  rc = put_plane(tmp, ef, n, on_device);
+  launch_fill_i32(tmp, n, 0, 0);
+  rc = put_plane(g, g->inst[inst], tmp, ef, on_device);
   if (!rc) rc = validate(g, tmp, n, -JFC_MAX_TERMINAL_CAP, JFC_MAX_TERMINAL_CAP, "excess");
   if (!rc) {
     launch_excess_split(tmp, g->cs + o, g->ct + o, n, 0);
@@ -537,9 +609,9 @@ extern "C" int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int3
   if (!cap) return fail(JFC_ERR_INVALID, "null capacity pointer");
   if (arc < 0 || arc >= g->inst[inst].K)
     return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
-  const long long n = g->inst_nodes(inst);
+  const long long n = g->inst_stored(inst);
   int *dst = g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off;
-  if ((rc = put_plane(dst, cap, n, on_device))) return rc;
+  if ((rc = put_plane(g, g->inst[inst], dst, cap, on_device))) return rc;
   if ((rc = validate(g, dst, n, 0, JFC_MAX_NEIGHBOR_CAP, "neighbour capacity"))) {
     launch_fill_i32(dst, n, 0, 0);
     (void)hipDeviceSynchronize();
@@ -557,21 +629,23 @@ extern "C" int jfc_set_seeds(jfc_graph *g, int inst, const uint8_t *fg, const ui
                              int on_device) {
   int rc;
   if ((rc = check_inst(g, inst))) return rc;
-  const long long n = g->inst_nodes(inst);
-  const long long o = g->inst[inst].node_off;
-  const unsigned char *dfg = fg, *dbg = bg;
+  const InstDesc &I = g->inst[inst];
+  const long long n = g->inst_stored(inst);
+  const long long o = I.node_off;
+  // seeds staged in the stored row layout (pad entries unseeded)
+  const unsigned char *dfg = nullptr, *dbg = nullptr;
   unsigned char *tfg = nullptr, *tbg = nullptr;
-  if (!on_device) {
-    if (fg) {
-      if ((rc = dalloc(&tfg, n, "seed staging"))) return rc;
-      JFC_HIP(hipMemcpy(tfg, fg, n, hipMemcpyHostToDevice));
-      dfg = tfg;
-    }
-    if (bg) {
-      if ((rc = dalloc(&tbg, n, "seed staging"))) { dfree(tfg); return rc; }
-      JFC_HIP(hipMemcpy(tbg, bg, n, hipMemcpyHostToDevice));
-      dbg = tbg;
-    }
+  if (fg) {
+    if ((rc = dalloc(&tfg, n, "seed staging"))) return rc;
+    launch_fill_u8(tfg, n, 0, 0);
+    if ((rc = put_rows(g, I, tfg, fg, 1, on_device))) { dfree(tfg); return rc; }
+    dfg = tfg;
+  }
+  if (bg) {
+    if ((rc = dalloc(&tbg, n, "seed staging"))) { dfree(tfg); return rc; }
+    launch_fill_u8(tbg, n, 0, 0);
+    if ((rc = put_rows(g, I, tbg, bg, 1, on_device))) { dfree(tfg); dfree(tbg); return rc; }
+    dbg = tbg;
   }
   launch_seeds(g->cs + o, g->ct + o, dfg, dbg, n, 0);
   hipError_t err = hipDeviceSynchronize();
@@ -1010,8 +1084,8 @@ extern "C" int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_de
   if ((rc = check_inst(g, inst))) return rc;
   if (!labels) return fail(JFC_ERR_INVALID, "null output");
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
-  JFC_HIP(hipMemcpy(labels, g->lab + g->inst[inst].node_off, g->inst_nodes(inst),
-                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+  if ((rc = get_rows(g, g->inst[inst], labels, g->lab + g->inst[inst].node_off, 1, on_device)))
+    return rc;
   return JFC_OK;
 }
 
@@ -1022,11 +1096,9 @@ extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *
   if (!fg || !tags) return fail(JFC_ERR_INVALID, "null foreground or output");
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   if (!g->tag && (rc = dalloc(&g->tag, (size_t)g->Ntot, "tag plane"))) return rc;
-  const long long n = g->inst_nodes(inst), o = g->inst[inst].node_off;
-  const hipMemcpyKind in = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  const hipMemcpyKind out = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  const long long o = g->inst[inst].node_off;
   launch_fill_u8(g->tag, g->Ntot, 0, 0);
-  JFC_HIP(hipMemcpyAsync(g->tag + o, fg, (size_t)n, in, 0));
+  if ((rc = put_rows(g, g->inst[inst], g->tag + o, fg, 1, on_device))) return rc;
   Solver S{};
   S.g = g;
   jfc_default_options(&S.o);
@@ -1036,8 +1108,7 @@ extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *
   int passes = 0;
   long long tiles = 0;
   if ((rc = S.tag_pass(&passes, &tiles))) return rc;
-  JFC_HIP(hipMemcpy(tags, g->tag + o, (size_t)n, out));
-  return JFC_OK;
+  return get_rows(g, g->inst[inst], tags, g->tag + o, 1, on_device);
 }
 
 extern "C" int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy) {
@@ -1060,10 +1131,9 @@ extern "C" int jfc_get_terminal_caps(jfc_graph *g, int inst, int32_t *cap_source
                                      int32_t *cap_sink, int on_device) {
   int rc;
   if ((rc = check_inst(g, inst))) return rc;
-  const long long n = g->inst_nodes(inst), o = g->inst[inst].node_off;
-  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  if (cap_source) JFC_HIP(hipMemcpy(cap_source, g->cs + o, n * 4, k));
-  if (cap_sink) JFC_HIP(hipMemcpy(cap_sink, g->ct + o, n * 4, k));
+  const InstDesc &I = g->inst[inst];
+  if (cap_source && (rc = get_rows(g, I, cap_source, g->cs + I.node_off, 4, on_device))) return rc;
+  if (cap_sink && (rc = get_rows(g, I, cap_sink, g->ct + I.node_off, 4, on_device))) return rc;
   return JFC_OK;
 }
 
@@ -1074,10 +1144,8 @@ extern "C" int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *c
   if (!cap) return fail(JFC_ERR_INVALID, "null output");
   if (arc < 0 || arc >= g->inst[inst].K)
     return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
-  const long long n = g->inst_nodes(inst);
-  JFC_HIP(hipMemcpy(cap, g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off, n * 4,
-                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
-  return JFC_OK;
+  return get_rows(g, g->inst[inst], cap, g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off,
+                  4, on_device);
 }
 
 extern "C" int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims,
@@ -1171,7 +1239,7 @@ extern "C" int jfc_get_tile_shape(const jfc_graph *g, int *shape) {
 
 extern "C" int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax) {
   if (!g) return fail(JFC_ERR_INVALID, "null graph");
-  if (nodes) *nodes = g->Ntot;
+  if (nodes) *nodes = g->Nuser;
   if (kmax) *kmax = g->Kg;
   return JFC_OK;
 }
@@ -1180,10 +1248,10 @@ extern "C" int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_
                                 float *d_neighbor, int on_device) {
   if (!g) return fail(JFC_ERR_INVALID, "null graph");
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
-  const long long n = g->Ntot;
+  const long long n = g->Nuser;
   if (on_device) {
     launch_backward_all(g->shape, g->tiles(), g->planes(), d_cap_source, d_cap_sink,
-                        d_neighbor, 0);
+                        d_neighbor, n, 0);
     JFC_HIP(hipDeviceSynchronize());
     return JFC_OK;
   }
@@ -1196,7 +1264,7 @@ extern "C" int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_
     return rc;
   }
   if (c) JFC_HIP(hipMemcpy(c, d_neighbor, n * g->Kg * 4, hipMemcpyHostToDevice));
-  launch_backward_all(g->shape, g->tiles(), g->planes(), a, b, c, 0);
+  launch_backward_all(g->shape, g->tiles(), g->planes(), a, b, c, n, 0);
   hipError_t err = hipDeviceSynchronize();
   if (err == hipSuccess && a) err = hipMemcpy(d_cap_source, a, n * 4, hipMemcpyDeviceToHost);
   if (err == hipSuccess && b) err = hipMemcpy(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost);

@@ -83,7 +83,7 @@ __device__ __forceinline__ TileRef tile_ref(const Tiles &T, int t) {
 
 __device__ __forceinline__ long long node_index(const InstDesc &I, int x, int y,
                                                 int z) {
-  return I.node_off + x + (long long)I.nx * (y + (long long)I.ny * z);
+  return I.node_off + x + (long long)I.px * (y + (long long)I.ny * z);
 }
 
 template <class S>

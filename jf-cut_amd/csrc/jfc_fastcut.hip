@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
     const int extent = AXIS == 0 ? I.nx - x0 : (AXIS == 1 ? I.ny - y0 : I.nz - z0);
     const int n = min(LEN, extent);
     const long long v0 = node_index(I, x0, y0, z0);
-    const long long step = AXIS == 0 ? 1 : (AXIS == 1 ? (long long)I.nx : (long long)I.nx * I.ny);
+    const long long step = AXIS == 0 ? 1 : (AXIS == 1 ? (long long)I.px : (long long)I.px * I.ny);
     int e[LEN], e0[LEN], h[LEN], cf[LEN], cb[LEN], cf0[LEN], cb0[LEN];
     for (int k = 0; k < n; ++k) {
       const long long v = v0 + k * step;

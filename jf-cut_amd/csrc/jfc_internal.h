@@ -19,6 +19,11 @@ constexpr int RED_CONST = 0, RED_DEF0 = 1, RED_ENERGY = 3, RED_SUP0 = 4,
 // One grid instance inside a graph.  All instances share the plane arrays;
 // instance i owns nodes [node_off, node_off + nx*ny*nz) and tiles
 // [tile_off, tile_off + tx*ty*tz).
+// Node v of an instance lives at node_off + x + px * (y + ny * z): rows are
+// padded to px, a multiple of 32 nodes, so every 32-node tile row is one
+// 128-byte line.  Pad nodes have no arcs and no terminal capacity.  Arrays
+// exchanged with callers use the unpadded numbering x + nx * (y + ny * z)
+// (user_off: the instance's first node in the graph's concatenation).
 struct InstDesc {
   int nx, ny, nz;
   int conn, K;
@@ -26,6 +31,8 @@ struct InstDesc {
   int tile_off;
   int ntiles;
   long long node_off;
+  int px;
+  long long user_off;
 };
 
 struct Planes {
@@ -151,6 +158,6 @@ void launch_backward(int shape, const Tiles &T, const Planes &P,
                      const InstDesc &I, float *dcs, float *dct, float *dnb,
                      hipStream_t st);
 void launch_backward_all(int shape, const Tiles &T, const Planes &P, float *dcs,
-                         float *dct, float *dnb, hipStream_t st);
+                         float *dct, float *dnb, long long user_total, hipStream_t st);
 
 }  // namespace jfc

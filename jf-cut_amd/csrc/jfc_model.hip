@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void k_km_assign(const void *img, int bpc, int
 
 // cs = round(scale * d2_bg / den), ct = round(scale * d2_fg / den); seeds get
 // (2^30, 0) / (0, 2^30).
-__global__ void k_unary(const void *img, int bpc, int C, long long n,
+// v: the image's (unpadded) node index; the planes store row r at r * px.
+__global__ void k_unary(const void *img, int bpc, int C, long long n, int nx, int px,
                         const unsigned char *seeds, const long long *cfg, int Kf,
                         const long long *cbg, int Kb, long long den, long long scale,
                         int *cs, int *ct) {
@@ -120,16 +121,17 @@ __global__ void k_unary(const void *img, int bpc, int C, long long n,
   __syncthreads();
   for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
        v += (long long)gridDim.x * blockDim.x) {
+    const long long sv = (v / nx) * px + v % nx;
     const int sd = seeds ? seeds[v] : 0;
-    if (sd == 1) { cs[v] = 1 << 30; ct[v] = 0; continue; }
-    if (sd == 2) { cs[v] = 0; ct[v] = 1 << 30; continue; }
+    if (sd == 1) { cs[sv] = 1 << 30; ct[sv] = 0; continue; }
+    if (sd == 2) { cs[sv] = 0; ct[sv] = 1 << 30; continue; }
     int x[3];
     for (int c = 0; c < C; ++c) x[c] = feat(img, bpc, v * C + c);
     long long df = km_d2(x, s_f, C), db = km_d2(x, s_b, C);
     for (int k = 1; k < Kf; ++k) df = min(df, km_d2(x, s_f + k * C, C));
     for (int k = 1; k < Kb; ++k) db = min(db, km_d2(x, s_b + k * C, C));
-    cs[v] = (int)((scale * db + den / 2) / den);
-    ct[v] = (int)((scale * df + den / 2) / den);
+    cs[sv] = (int)((scale * db + den / 2) / den);
+    ct[sv] = (int)((scale * df + den / 2) / den);
   }
 }
 
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void k_pairwise(InstDesc I, const void *img, i
             else hi = mid;
           }
         }
-        c0[d * stride + I.node_off + v] = w;
+        c0[d * stride + I.node_off + x + (long long)I.px * row] = w;
       }
     }
   }
@@ -378,7 +380,7 @@ extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int ch
     MODEL_HIP(hipMemcpy(d_cf, cent_side[0].data(), cent_side[0].size() * sizeof(long long), hipMemcpyHostToDevice));
     MODEL_HIP(hipMemcpy(d_cb, cent_side[1].data(), cent_side[1].size() * sizeof(long long), hipMemcpyHostToDevice));
     const long long den = prm->range * prm->range * (1LL << (2 * KM_FRAC));
-    k_unary<<<grid1(n), 256>>>(img, bpc, C, n, sd, d_cf, prm->k_fg, d_cb, prm->k_bg, den,
+    k_unary<<<grid1(n), 256>>>(img, bpc, C, n, I.nx, I.px, sd, d_cf, prm->k_fg, d_cb, prm->k_bg, den,
                                prm->unary_scale, cs + I.node_off, ct + I.node_off);
     // pairwise terms
     MODEL_HIP(hipMemset(d_sums, 0, 4 * sizeof(long long)));

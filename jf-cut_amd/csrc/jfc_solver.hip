@@ -119,7 +119,7 @@ __global__ void k_zero_border(InstDesc I, int *plane, int d) {
     const int z = (int)(r / I.ny);
     const int X = x + dx, Y = y + dy, Z = z + dz;
     if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz)
-      plane[I.node_off + i] = 0;
+      plane[node_index(I, x, y, z)] = 0;
   }
 }
 
@@ -244,7 +244,7 @@ __device__ __forceinline__ RowRef<S> row_ref(const Tiles &T, int r) {
   RowRef<S> R;
   R.inst = lo;
   R.nx = I.nx;
-  R.v0 = I.node_off + (long long)rr * I.nx;
+  R.v0 = I.node_off + (long long)rr * I.px;
   R.tile0 = I.tile_off + I.tx * (y / S::TH + I.ty * (z / S::TD));
   return R;
 }
@@ -585,7 +585,7 @@ template <class S>
 __global__ __launch_bounds__(256) void k_backward(Tiles T, Planes P, int t_begin,
                                                   int t_count, float *dcs,
                                                   float *dct, float *dnb,
-                                                  int all_instances) {
+                                                  long long user_total) {
   const Lane<S> L(threadIdx.x);
   for (int t = t_begin + blockIdx.x; t < t_begin + t_count; t += gridDim.x) {
     const TileRef r = tile_ref<S>(T, t);
@@ -593,10 +593,12 @@ __global__ __launch_bounds__(256) void k_backward(Tiles T, Planes P, int t_begin
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     if (!(gx < I.nx && gy < I.ny && gz < I.nz)) continue;
     const long long v = node_index(I, gx, gy, gz);
-    // one instance: its own numbering and plane stride; all instances: the
-    // graph's concatenated numbering with plane stride = total nodes
-    const long long lv = all_instances ? v : v - I.node_off;
-    const long long n = all_instances ? P.stride : (long long)I.nx * I.ny * I.nz;
+    // outputs in the callers' numbering: one instance (user_total = 0) its
+    // own, with plane stride = its node count; all instances the graph's
+    // concatenation, with plane stride = all their nodes
+    const long long own = gx + (long long)I.nx * (gy + (long long)I.ny * gz);
+    const long long lv = user_total ? I.user_off + own : own;
+    const long long n = user_total ? user_total : (long long)I.nx * I.ny * I.nz;
     const bool s = P.lab[v] != 0;
     if (dcs) dcs[lv] = s ? 0.f : 1.f;
     if (dct) dct[lv] = s ? 1.f : 0.f;
@@ -624,9 +626,9 @@ void launch_backward(int shape, const Tiles &T, const Planes &P,
 }
 
 void launch_backward_all(int shape, const Tiles &T, const Planes &P, float *dcs,
-                         float *dct, float *dnb, hipStream_t st) {
+                         float *dct, float *dnb, long long user_total, hipStream_t st) {
   JFC_DISPATCH(shape, (k_backward<S><<<tile_grid(T.ntiles), 256, 0, st>>>(
-                          T, P, 0, T.ntiles, dcs, dct, dnb, 1)));
+                          T, P, 0, T.ntiles, dcs, dct, dnb, user_total)));
 }
 
 // ---- initialisation of relabel / cut searches --------------------------------

@@ -159,7 +159,7 @@ template <bool LAZY>
 __global__ __launch_bounds__(256) void k_wm_unary(
     const uchar4 *img, const unsigned char *fg, const unsigned char *bg, long long n, W4 w,
     const float *cent, int k, const unsigned *hist, unsigned sum_f, unsigned sum_b, int shift,
-    float lambda, int *cs, int *ct) {
+    float lambda, int nx, int px, int *cs, int *ct) {
   __shared__ float s_c[2 * kMaxK * 4];
   __shared__ unsigned s_h[2 * kLevels];
   if (LAZY)
@@ -201,8 +201,10 @@ __global__ __launch_bounds__(256) void k_wm_unary(
       }
       ef = unary_int(val);
     }
-    cs[v] = ef > 0 ? ef : 0;
-    ct[v] = ef < 0 ? -ef : 0;
+    // the planes store row r at r * px
+    const long long sv = (v / nx) * px + v % nx;
+    cs[sv] = ef > 0 ? ef : 0;
+    ct[sv] = ef < 0 ? -ef : 0;
   }
 }
 
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(256) void k_wm_pairwise(InstDesc I, const uchar4 *i
           scaled_texel(img, X + (long long)I.nx * (Y + (long long)I.ny * Z), w, b);
           cap = wm_arc<LAZY, D3>(a, b, scale);
         }
-        c0[d * stride + I.node_off + v] = cap;
+        c0[d * stride + I.node_off + x + (long long)I.px * row] = cap;
       }
     }
   }
@@ -454,11 +456,13 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
       WM_HIP(hipMemcpy(d_cent, cent_used.data(), cent_used.size() * sizeof(float),
                        hipMemcpyHostToDevice));
       k_wm_unary<true><<<grid1(n), 256>>>(img, dfg, dbg, n, w, d_cent, k, nullptr, 0, 0, shift,
-                                          prm->lazy_terminal, cs + I.node_off, ct + I.node_off);
+                                          prm->lazy_terminal, I.nx, I.px, cs + I.node_off,
+                                          ct + I.node_off);
     } else {
       k_wm_unary<false><<<grid1(n), 256>>>(img, dfg, dbg, n, w, nullptr, 0, d_hist,
                                            ST.hist_sum[0], ST.hist_sum[1], shift,
-                                           prm->bk_terminal, cs + I.node_off, ct + I.node_off);
+                                           prm->bk_terminal, I.nx, I.px, cs + I.node_off,
+                                           ct + I.node_off);
     }
     WM_HIP(hipGetLastError());
     const int grid = (int)std::min<long long>((long long)I.ny * I.nz, 65536);
