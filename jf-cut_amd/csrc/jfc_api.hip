@@ -731,19 +731,19 @@ struct Solver {
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
     long long before = 0;
     if (int rc = read_ll(total, &before)) return rc;
-    auto flood = [&](const Queue &q) {
+    auto flood = [&](const Queue &q, bool full_pass) {
       if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
       else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
-      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, st);
+      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, full_pass, st);
     };
     if (mode != kRelabel) {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
       launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, q, nullptr, total, st);
-      flood(q);
+      flood(q, false);
     } else {
-      flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles});
+      flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles}, true);
     }
     advance();
     *passes += 1;
@@ -754,7 +754,7 @@ struct Solver {
         Queue qc = queue_from_list();
         launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, qc, nullptr, total,
                        st);
-        flood(qc);
+        flood(qc, false);
         advance();
         *passes += 1;
       }

@@ -141,7 +141,7 @@ void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const 
                      unsigned char *flag, hipStream_t st);
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  hipStream_t st);
+                  bool full_pass, hipStream_t st);
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st);

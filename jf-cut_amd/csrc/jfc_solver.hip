@@ -1019,9 +1019,12 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // its groups' runs in order.  Neighbouring tiles of a jump-flooding band
 // thus stay on one XCD's L2, and claims spread over 8 addresses.  Every run
 // belongs to one head and every head has workgroups, so the list is always
-// drained.  Slot s of `ctr` holds the list count at [QSLOT*s] and the heads
-// at [QSLOT*s + 8 ...]; the consumer of slot s zeroes the other slot for the
-// next producer.
+// drained.  When the static first runs already cover the list (G * chunk >=
+// count, the short lists of late sweeps and relabel passes) no claim is
+// issued at all: thousands of workgroups claiming on 8 addresses would
+// serialise there.  Slot s of `ctr` holds the list count at [QSLOT*s] and the
+// heads at [QSLOT*s + 8 ...]; the consumer of slot s zeroes the other slot
+// for the next producer.
 struct QueueCursor {
   int count, chunk, first, head_id, nheads;
   int ahead;   // (lane 0) claim for the run after the current one
@@ -1058,13 +1061,18 @@ struct QueueCursor {
     if (idx >= 0) {
       if (threadIdx.x == 0) *s_run = ahead;
       __syncthreads();
-      const int k = *s_run;  // k-th run handed out by this head
+      const int k = *s_run;  // k-th run handed out by this head (< 0: none)
+      if (k < 0) {
+        idx = end = count;
+        return false;
+      }
       run = (int)gridDim.x + 8 * (head_id + nheads * (k >> 3)) + (k & 7);
     }
     idx = run * chunk;
     end = min(idx + chunk, count);
     const bool ok = idx < count;
-    if (ok && threadIdx.x == 0) ahead = atomicAdd(&heads[head_id], 1);
+    if (ok && threadIdx.x == 0)
+      ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[head_id], 1) : -1;
     return ok;
   }
 };
@@ -1074,11 +1082,11 @@ struct QueueCursor {
 // still runs its static first run after the others are done, stretching the
 // launch -- so the grid is capped at what the device holds at once for this
 // kernel (occupancy per CU, from its register and LDS use, times CUs).
-template <auto Kernel>
-static int resident_grid(int want) {
+template <auto Kernel, int Block = 256>
+static int resident_grid_b(int want) {
   static const int per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, Kernel, 256, 0) != hipSuccess || n < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, Kernel, Block, 0) != hipSuccess || n < 1)
       n = 1;
     return n;
   }();
@@ -1088,6 +1096,11 @@ static int resident_grid(int want) {
       cus < 1)
     cus = 1;
   return std::max(1, std::min(want, per_cu * cus));
+}
+
+template <auto Kernel>
+static int resident_grid(int want) {
+  return resident_grid_b<Kernel, 256>(want);
 }
 
 // ---- residual-arc masks and lazy residual planes ----------------------------------
@@ -1415,10 +1428,175 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
   }
 }
 
+// One wavefront per tile.  A relax visit is bound by its load latency (the
+// local fixpoint is a few cheap LDS iterations), so the tiles in flight per
+// CU matter more than the lanes per tile: a 64-lane wave owning a tile keeps
+// up to four times as many tiles in flight as a 256-lane workgroup.  Lane l
+// holds the nodes l + 64 j; the iterations are ordered by program order in
+// the wave, and a lane's later nodes already see its earlier nodes' new
+// heights (Gauss-Seidel within the iteration, still monotone).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cells of the padded tile a connectivity reads outside the tile: the sides
+// and corners in 2D, the faces in 3D.
+template <class S>
+struct Ring {
+  static constexpr int N = S::D3 ? 2 * (S::TW * S::TH + S::TW * S::TD + S::TH * S::TD)
+                                 : 2 * S::PW + 2 * S::TH;
+  static __device__ __forceinline__ int pos(int k) {
+    if constexpr (!S::D3) {
+      if (k < S::PW) return k;
+      k -= S::PW;
+      if (k < S::PW) return (S::PH - 1) * S::PW + k;
+      k -= S::PW;
+      if (k < S::TH) return (k + 1) * S::PW;
+      k -= S::TH;
+      return (k + 1) * S::PW + S::PW - 1;
+    } else {
+      constexpr int XY = S::TW * S::TH, XZ = S::TW * S::TD, YZ = S::TH * S::TD;
+      int x, y, z;
+      if (k < 2 * XY) {
+        const int q = k % XY;
+        x = q % S::TW + 1;
+        y = q / S::TW + 1;
+        z = k < XY ? 0 : S::PD - 1;
+      } else if (k < 2 * XY + 2 * XZ) {
+        k -= 2 * XY;
+        const int q = k % XZ;
+        x = q % S::TW + 1;
+        z = q / S::TW + 1;
+        y = k < XZ ? 0 : S::PH - 1;
+      } else {
+        k -= 2 * XY + 2 * XZ;
+        const int q = k % YZ;
+        y = q % S::TH + 1;
+        z = q / S::TH + 1;
+        x = k < YZ ? 0 : S::PW - 1;
+      }
+      return x + S::PW * (y + S::PH * z);
+    }
+  }
+};
+
+template <class S>
+__global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
+                                                unsigned char *dirty, unsigned char *active) {
+  constexpr int NJ = S::NT / 64;
+  constexpr int NR = (Ring<S>::N + 63) / 64;
+  __shared__ int s_h[S::PADN];
+  __shared__ int s_run;
+  const int lane = threadIdx.x;
+  QueueCursor Qc(q, &s_run);
+  int idx = -1, end = 0;
+  while (Qc.next(idx, end)) {
+    const int t = q.list[idx];
+    const TileRef r = tile_ref<S>(T, t);
+    const InstDesc &I = r.I;
+    const int K = I.K;
+    // loads: ring heights (device-coherent, as in k_relax) and the lane's nodes
+    int hh[NR];
+    bool hlive[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int k = lane + 64 * i;
+      const HaloCell hc =
+          halo_cell<S>(I, r.ox, r.oy, r.oz, k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN, K);
+      hlive[i] = hc.live;
+      hh[i] = __hip_atomic_load(&P.h[hc.g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    int h0[NJ];
+    unsigned mk[NJ];
+    bool pos[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const Lane<S> L(lane + 64 * j);
+      const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+      const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
+      const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
+      h0[j] = P.h[gv];
+      mk[j] = P.rmask[gv] & ((1u << K) - 1u);
+      pos[j] = P.e[gv] > 0;
+      if (!valid) {
+        h0[j] = HINF;
+        mk[j] = 0;
+        pos[j] = false;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) s_h[Lane<S>(lane + 64 * j).me] = h0[j];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int k = lane + 64 * i;
+      if (k < Ring<S>::N) s_h[Ring<S>::pos(k)] = hlive[i] ? hh[i] : HINF;
+    }
+    wave_lds_sync();
+    bool any;
+    do {
+      bool ch = false;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        if (!mk[j]) continue;
+        const int me = Lane<S>(lane + 64 * j).me;
+        const int h = s_h[me];
+        int m = h;
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d)
+          if (mk[j] & (1u << d)) m = min(m, s_h[me + lds_off<S>(d)] + 1);
+        if (m < h) {
+          s_h[me] = m;
+          ch = true;
+        }
+      }
+      any = __ballot(ch) != 0;
+      wave_lds_sync();
+    } while (any);
+    bool act = false;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const Lane<S> L(lane + 64 * j);
+      const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
+      if (!(gx < I.nx && gy < I.ny && gz < I.nz)) continue;
+      const int h1 = s_h[L.me];
+      act |= pos[j] && h1 < HINF;
+      if (h1 == h0[j]) continue;
+      __hip_atomic_store(&P.h[node_index(I, gx, gy, gz)], h1, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) {
+        if (d >= K) break;
+        int dx, dy, dz;
+        dir_of<S::D3>(d, dx, dy, dz);
+        const int lx = L.lx + dx, ly = L.ly + dy, lz = L.lz + dz;
+        if (lx >= 0 && lx < S::TW && ly >= 0 && ly < S::TH && lz >= 0 && lz < S::TD) continue;
+        const int X = gx + dx, Y = gy + dy, Z = gz + dz;
+        if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) continue;
+        dirty[tile_of<S>(I, X, Y, Z)] = 1;
+      }
+    }
+    const bool a = __ballot(act) != 0;
+    if (lane == 0) active[t] = (unsigned char)a;
+    wave_lds_sync();
+  }
+}
+
+// 2D tiles: the wave-per-tile kernel (config 2's relabel 1.67 -> 1.21 ms).
+// 3D tiles iterate more per visit (fronts cross 8x8x4 tiles in z as well)
+// and there the 256-lane visit's four parallel waves win (config 4's
+// relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  hipStream_t st) {
-  JFC_DISPATCH(shape, (k_relax<S><<<resident_grid<&k_relax<S>>(nwg), 256, 0, st>>>(
+                  bool full_pass, hipStream_t st) {
+  (void)full_pass;
+  if (shape_is_3d(shape)) {
+    JFC_DISPATCH(shape, (k_relax<S><<<resident_grid<&k_relax<S>>(nwg), 256, 0, st>>>(
+                            T, P, q, dirty, active)));
+    return;
+  }
+  JFC_DISPATCH(shape, (k_relax_w<S><<<resident_grid_b<&k_relax_w<S>, 64>(4 * nwg), 64, 0, st>>>(
                           T, P, q, dirty, active)));
 }
 
