@@ -67,7 +67,9 @@ typedef struct {
   double relabel_work;    /* run an exact global relabel once the tiles visited
                              since the last one exceed this many times the tile
                              count (0 = only at start and on certificate failure) */
-  int max_local_iters;    /* cap on LDS discharge iterations per tile visit */
+  int max_local_iters;    /* cap on LDS discharge iterations per tile visit
+                             (0 = the cap jfc_autotune chose for the graph's
+                             size class, else 2) */
   int tile_shape;         /* JFC_TILE_* (0 = default for the dimensionality) */
   int profile;            /* 1 = time every push-relabel launch with HIP events */
   void *stream;           /* hipStream_t to issue work on (NULL = default stream) */
@@ -237,17 +239,22 @@ int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
  * the tile size from the UI, QGCW:826-835) ---------------------------------- */
 
 /* Time `reps` solves of the graph (after one warm-up) with every tile shape of
- * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2) under the other
- * options of opt (may be NULL), and keep the fastest as the shape the graph's
- * solves use when opt->tile_shape is JFC_TILE_DEFAULT.  The choice is also
- * stored for the graph's size class (dimensionality, connectivities, instance
- * count and mean instance size, each rounded to a power of two): graphs of the
- * class created later in the process start with it.  Results are unchanged by
- * the shape.  *chosen (may be NULL) receives the JFC_TILE_* value. */
+ * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2) and, unless
+ * opt->max_local_iters fixes it, every local-iteration cap of {2, 3, 4}, under
+ * the other options of opt (may be NULL), and keep the fastest pair as what
+ * the graph's solves use when opt->tile_shape is JFC_TILE_DEFAULT and
+ * opt->max_local_iters is 0.  The choice is also stored for the graph's size
+ * class (dimensionality, connectivities, instance count and mean instance
+ * size, each rounded to a power of two): graphs of the class created later in
+ * the process start with it.  Results are unchanged by either choice.
+ * *chosen (may be NULL) receives the JFC_TILE_* value. */
 int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int *chosen);
 
 /* The JFC_TILE_* shape a solve with tile_shape = JFC_TILE_DEFAULT uses. */
 int jfc_get_tile_shape(const jfc_graph *g, int *shape);
+
+/* The local-iteration cap a solve with max_local_iters = 0 uses. */
+int jfc_get_local_iters(const jfc_graph *g, int *max_local_iters);
 
 /* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
 

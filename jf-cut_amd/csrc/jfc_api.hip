@@ -86,8 +86,13 @@ int shape_from_option(int opt, int dim) {
 }
 
 // Process-wide autotuning table: size class -> JFC_TILE_* shape.
+// Autotuned choices per size class: tile shape and local-iteration cap.
+struct Tuned {
+  int shape = 0;
+  int mli = 0;
+};
 std::mutex g_tune_mu;
-std::map<std::string, int> g_tune;
+std::map<std::string, Tuned> g_tune;
 
 int log2_round(double v) { return v <= 1.0 ? 0 : (int)std::lround(std::log2(v)); }
 
@@ -106,6 +111,7 @@ long long push_bytes_per_node(int K) { return 2LL * 4 * (2 + K); }
 struct jfc_graph {
   int dim = 2;
   int tuned = 0;           // JFC_TILE_* chosen by jfc_autotune (0 = none)
+  int tuned_mli = 0;       // local-iteration cap chosen by jfc_autotune (0 = none)
   std::string size_class;  // key of the process-wide autotuning table
   int Kg = 4;  // planes stored (max K over instances)
   int shape = 0;
@@ -336,7 +342,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->max_sweeps = 1 << 20;
   o->check_every = 1;
   o->relabel_work = 2.0;
-  o->max_local_iters = 2;
+  o->max_local_iters = 0;
   o->tile_shape = JFC_TILE_DEFAULT;
   o->profile = 0;
   o->stream = nullptr;
@@ -445,7 +451,10 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
                     std::to_string(log2_round(count)) + ":" + std::to_string(log2_round(mean));
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(g->size_class);
-    if (it != g_tune.end()) g->tuned = it->second;
+    if (it != g_tune.end()) {
+      g->tuned = it->second.shape;
+      g->tuned_mli = it->second.mli;
+    }
   }
   if ((rc = g->set_shape(default_shape(g->dim)))) { delete g; return rc; }
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1023,7 +1032,8 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.max_sweeps <= 0) S.o.max_sweeps = 1 << 20;
   if (S.o.max_sweeps > (1 << 24)) S.o.max_sweeps = 1 << 24;
   if (S.o.check_every <= 0) S.o.check_every = 1;
-  if (S.o.max_local_iters <= 0) S.o.max_local_iters = 1;
+  // 0: the cap jfc_autotune chose for the graph's size class, else 2
+  if (S.o.max_local_iters <= 0) S.o.max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
   if (S.o.tile_order != 0 && S.o.tile_order != 1)
     return fail(JFC_ERR_INVALID, "tile_order must be 0 (distance transform) or 1 (jump flooding)");
   if (S.o.method != JFC_METHOD_JFCUT && S.o.method != JFC_METHOD_FASTCUT)
@@ -1204,28 +1214,38 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   const int cand3[2] = {JFC_TILE_8x8x4, JFC_TILE_32x4x2};
   const int *cand = g->dim == 3 ? cand3 : cand2;
   const int nc = g->dim == 3 ? 2 : 3;
-  int best = 0;
+  // the local-iteration cap is tuned too unless the caller fixed it: fronts
+  // that cross tiles in few visits favour more iterations per visit
+  // (config 4: 26 -> 20 ms at 4), others favour 2 (config 3: 17 vs 31 ms)
+  const int mli_fixed = o.max_local_iters > 0 ? o.max_local_iters : 0;
+  const int mcand[3] = {2, 3, 4};
+  const int nm = mli_fixed ? 1 : 3;
+  int best = 0, best_mli = 0;
   double best_ms = 0.0;
-  for (int c = 0; c < nc; ++c) {
-    o.tile_shape = cand[c];
-    std::vector<double> ms;
-    for (int r = 0; r <= reps; ++r) {
-      jfc_stats st;
-      const int rc = jfc_solve(g, &o, &st);
-      if (rc < 0) return rc;
-      if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+  for (int c = 0; c < nc; ++c)
+    for (int m = 0; m < nm; ++m) {
+      o.tile_shape = cand[c];
+      o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
+      std::vector<double> ms;
+      for (int r = 0; r <= reps; ++r) {
+        jfc_stats st;
+        const int rc = jfc_solve(g, &o, &st);
+        if (rc < 0) return rc;
+        if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+      }
+      std::sort(ms.begin(), ms.end());
+      const double med = ms[ms.size() / 2];
+      if (!best || med < best_ms) {
+        best = cand[c];
+        best_mli = o.max_local_iters;
+        best_ms = med;
+      }
     }
-    std::sort(ms.begin(), ms.end());
-    const double med = ms[ms.size() / 2];
-    if (!best || med < best_ms) {
-      best = cand[c];
-      best_ms = med;
-    }
-  }
   g->tuned = best;
+  g->tuned_mli = mli_fixed ? 0 : best_mli;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[g->size_class] = best;
+    g_tune[g->size_class] = Tuned{best, g->tuned_mli};
   }
   if (chosen) *chosen = best;
   return JFC_OK;
@@ -1234,6 +1254,12 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
 extern "C" int jfc_get_tile_shape(const jfc_graph *g, int *shape) {
   if (!g || !shape) return fail(JFC_ERR_INVALID, "null argument");
   *shape = g->tuned ? g->tuned : (g->dim == 3 ? JFC_TILE_8x8x4 : JFC_TILE_32x8);
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_local_iters(const jfc_graph *g, int *max_local_iters) {
+  if (!g || !max_local_iters) return fail(JFC_ERR_INVALID, "null argument");
+  *max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
   return JFC_OK;
 }
 

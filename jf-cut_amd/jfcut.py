@@ -99,7 +99,7 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
-           "jfc_autotune", "jfc_get_tile_shape", "jfc_default_widget_model_params",
+           "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_default_widget_model_params",
            "jfc_model_widget"]
 
 
@@ -156,6 +156,7 @@ def lib():
         "jfc_get_tags": (I, [P, I, P, P, I]),
         "jfc_autotune": (I, [P, ctypes.POINTER(Options), I, ctypes.POINTER(I)]),
         "jfc_get_tile_shape": (I, [P, ctypes.POINTER(I)]),
+        "jfc_get_local_iters": (I, [P, ctypes.POINTER(I)]),
         "jfc_default_widget_model_params": (None, [ctypes.POINTER(WidgetModelParams)]),
         "jfc_model_widget": (I, [P, I, P, P, P, I, ctypes.POINTER(WidgetModelParams),
                                  ctypes.POINTER(WidgetModelStats)]),
@@ -362,8 +363,9 @@ class GridGraph:
         return t
 
     def autotune(self, reps=2, **kw):
-        """Time every tile shape on this graph and keep the fastest
-        (jfc_autotune); returns its name."""
+        """Time every tile shape (and, unless max_local_iters is given, every
+        local-iteration cap of 2, 3, 4) on this graph and keep the fastest
+        (jfc_autotune); returns the shape's name."""
         o = default_options(**kw)
         ch = ctypes.c_int()
         _check(lib().jfc_autotune(self._h, ctypes.byref(o), reps, ctypes.byref(ch)))
@@ -374,6 +376,12 @@ class GridGraph:
         ch = ctypes.c_int()
         _check(lib().jfc_get_tile_shape(self._h, ctypes.byref(ch)))
         return TILE_NAME[ch.value]
+
+    def local_iters(self):
+        """Local-iteration cap default solves (max_local_iters=0) use."""
+        ch = ctypes.c_int()
+        _check(lib().jfc_get_local_iters(self._h, ctypes.byref(ch)))
+        return ch.value
 
     def size(self):
         """(total nodes over all instances, largest connectivity)."""

@@ -220,8 +220,9 @@ def test_batch_gpu(gpu):
 
 
 def test_autotune_gpu(gpu):
-    """jfc_autotune picks one of the shapes, leaves results unchanged, and a
-    later graph of the same size class starts with the same shape."""
+    """jfc_autotune picks one of the shapes and local-iteration caps, leaves
+    results unchanged, and a later graph of the same size class starts with
+    the same pair; a cap fixed by the caller is not tuned."""
     rng = np.random.default_rng(77)
     shapes = [(int(rng.integers(40, 120)), int(rng.integers(40, 120)), 1, 4 if i % 2 else 8)
               for i in range(6)]
@@ -237,8 +238,10 @@ def test_autotune_gpu(gpu):
 
     g = build()
     best = g.autotune(reps=1)
-    say("case autotune", best)
+    mli = g.local_iters()
+    say("case autotune", best, mli)
     assert best in ("32x8", "64x4", "16x16") and g.tile_shape() == best
+    assert mli in (2, 3, 4)
     st = g.solve()
     assert st.converged == 1
     for i, (cs, ct, caps) in enumerate(data):
@@ -246,7 +249,10 @@ def test_autotune_gpu(gpu):
         assert g.flow(i) == f and np.array_equal(g.labels(i), l), i
     g.close()
     g2 = build()
-    assert g2.tile_shape() == best
+    assert g2.tile_shape() == best and g2.local_iters() == mli
+    # a fixed cap: only the shape is tuned, and the class keeps no cap
+    best2 = g2.autotune(reps=1, max_local_iters=1)
+    assert best2 in ("32x8", "64x4", "16x16") and g2.local_iters() == 2
     g2.close()
 
 
