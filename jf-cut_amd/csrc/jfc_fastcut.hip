@@ -35,8 +35,8 @@ template <class S>
 __global__ __launch_bounds__(256) void k_fc_relabel(Tiles T, Planes P, Queue q, int parity,
                                                     int clear_other) {
   // the next compaction appends to the other slot
-  if (clear_other && blockIdx.x == 0 && threadIdx.x < QSLOT)
-    q.ctr[QSLOT * (q.slot ^ 1) + threadIdx.x] = 0;
+  if (clear_other && blockIdx.x == 0 && threadIdx.x < 9)
+    q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
   const int count = list_count(q);
   const Lane<S> L(threadIdx.x);
   for (int i = blockIdx.x; i < count; i += gridDim.x) {

@@ -69,10 +69,14 @@ bool shape_is_3d(int shape);
 void shape_dims(int shape, int *tw, int *th, int *td);
 
 // Persistent work queues.  Slot s of ctr (QSLOT ints) holds the list count at
-// [QSLOT*s] and the claim heads at [QSLOT*s + 8 ...]; every consumer zeroes
-// the other slot, so producer/consumer launches alternate slots without any
-// memset.
-constexpr int QSLOT = 16;
+// [QSLOT*s] and claim head h at [QSLOT*s + QWORD*(1+h)], h < 8; every consumer
+// zeroes the other slot's words, so producer/consumer launches alternate slots
+// without any memset.  The words sit 256 bytes apart: device atomics on one
+// cache line serialize, and with the eight heads in one line the claims of a
+// launch queued behind each other (config 2 solve 5.64 -> 5.22 ms, config 3
+// 16.9 -> 14.1 ms once they were split).
+constexpr int QWORD = 64;
+constexpr int QSLOT = 9 * QWORD;
 struct Queue {
   int *list;
   int *ctr;

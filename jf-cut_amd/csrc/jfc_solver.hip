@@ -1017,24 +1017,25 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // heads, head x serves the workgroups of XCD x: runs >= G form groups of 8
 // consecutive runs, group q belongs to head q % nheads, and a head hands out
 // its groups' runs in order.  Neighbouring tiles of a jump-flooding band
-// thus stay on one XCD's L2, and claims spread over 8 addresses.  Every run
+// thus stay on one XCD's L2, and claims spread over 8 addresses in 8
+// different cache lines.  Every run
 // belongs to one head and every head has workgroups, so the list is always
 // drained.  When the static first runs already cover the list (G * chunk >=
 // count, the short lists of late sweeps and relabel passes) no claim is
 // issued at all: thousands of workgroups claiming on 8 addresses would
-// serialise there.  Slot s of `ctr` holds the list count at [QSLOT*s] and the
-// heads at [QSLOT*s + 8 ...]; the consumer of slot s zeroes the other slot
-// for the next producer.
+// serialise there.  Slot s of `ctr` holds the list count at [QSLOT*s] and
+// head h at [QSLOT*s + QWORD*(1+h)]; the consumer of slot s zeroes the other
+// slot's words for the next producer.
 struct QueueCursor {
   int count, chunk, first, head_id, nheads;
   int ahead;   // (lane 0) claim for the run after the current one
   int *heads;
   int *s_run;  // LDS word broadcasting the claimed run
   __device__ __forceinline__ QueueCursor(const Queue &q, int *lds_word) {
-    if (blockIdx.x == 0 && threadIdx.x < QSLOT) q.ctr[QSLOT * (q.slot ^ 1) + threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 9) q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
     count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
     chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
-    heads = q.ctr + QSLOT * q.slot + 8;
+    heads = q.ctr + QSLOT * q.slot + QWORD;
     s_run = lds_word;
     const int b = blockIdx.x;
     nheads = (gridDim.x % 8 == 0) ? 8 : 1;
@@ -1072,7 +1073,7 @@ struct QueueCursor {
     end = min(idx + chunk, count);
     const bool ok = idx < count;
     if (ok && threadIdx.x == 0)
-      ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[head_id], 1) : -1;
+      ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[QWORD * head_id], 1) : -1;
     return ok;
   }
 };
