@@ -159,6 +159,8 @@ struct jfc_graph {
   int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
   hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
   hipEvent_t priv_event = nullptr;
+  int *h_cnt = nullptr;                 // pinned: last list counts of two pass groups
+  hipEvent_t cnt_ev[2] = {nullptr, nullptr};
   // Captured launch sequence of one sweep, per starting queue slot.
   struct SweepGraph {
     hipGraphExec_t exec = nullptr;
@@ -195,6 +197,12 @@ struct jfc_graph {
     if (priv_stream) (void)hipStreamDestroy(priv_stream);
     priv_event = nullptr;
     priv_stream = nullptr;
+    for (hipEvent_t &ev : cnt_ev) {
+      if (ev) (void)hipEventDestroy(ev);
+      ev = nullptr;
+    }
+    if (h_cnt) (void)hipHostFree(h_cnt);
+    h_cnt = nullptr;
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
@@ -713,11 +721,6 @@ struct Solver {
     JFC_HIP(hipStreamSynchronize(st));
     return JFC_OK;
   }
-  int read_ll(const long long *dptr, long long *out) {
-    JFC_HIP(hipMemcpyAsync(out, dptr, sizeof(long long), hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipStreamSynchronize(st));
-    return JFC_OK;
-  }
   // relabel / cut passes issued between two host checks
   static constexpr int kPassGroup = 4;
 
@@ -726,8 +729,9 @@ struct Solver {
   // Tile passes of a monotone search (relabel, cut or tag) until no tile is
   // dirty.  The first pass visits every tile (relabel) or the seed tiles (cut,
   // tag) in jump-flooding distance order; later passes visit the dirty tiles,
-  // alternating direction (back-and-forth, QGCWidget.cpp:2221-2240).
-  int search(int mode, int *passes, long long *tiles) {
+  // alternating direction (back-and-forth, QGCWidget.cpp:2221-2240).  Tile
+  // visits are counted on the device (d_total) and read once per solve.
+  int search(int mode, int *passes) {
     const Tiles T = g->tiles();
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
@@ -738,8 +742,10 @@ struct Solver {
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
-    long long before = 0;
-    if (int rc = read_ll(total, &before)) return rc;
+    if (!g->h_cnt) {
+      JFC_HIP(hipHostMalloc((void **)&g->h_cnt, 2 * sizeof(int), hipHostMallocDefault));
+      for (hipEvent_t &ev : g->cnt_ev) JFC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
     auto flood = [&](const Queue &q, bool full_pass) {
       if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
       else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
@@ -756,10 +762,15 @@ struct Solver {
     }
     advance();
     *passes += 1;
-    // later passes over the dirty tiles, alternating direction, issued in
-    // groups between host checks; an empty compaction ends the search
-    for (int pass = 1;;) {
-      for (int k = 0; k < kPassGroup; ++k, ++pass) {
+    // later passes over the dirty tiles, alternating direction, in groups;
+    // an empty compaction ends the search.  Each group ends by copying its
+    // last list count to pinned memory, and groups are issued one ahead of
+    // the host check, so the device runs group k+1 while the host waits for
+    // group k's count instead of idling through every check.  A group issued
+    // after convergence compacts empty lists and changes nothing.
+    int pass = 1;
+    auto issue_group = [&](int k) -> int {
+      for (int j = 0; j < kPassGroup; ++j, ++pass) {
         Queue qc = queue_from_list();
         launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, qc, nullptr, total,
                        st);
@@ -767,34 +778,36 @@ struct Solver {
         advance();
         *passes += 1;
       }
-      int cnt = 0;
       // the last compaction's slot is the one before the current
-      if (int rc = read_int(g->d_ctr + QSLOT * (g->slot ^ 1), &cnt)) return rc;
-      if (cnt == 0) break;
+      JFC_HIP(hipMemcpyAsync(g->h_cnt + (k & 1), g->d_ctr + QSLOT * (g->slot ^ 1), sizeof(int),
+                             hipMemcpyDeviceToHost, st));
+      JFC_HIP(hipEventRecord(g->cnt_ev[k & 1], st));
+      return JFC_OK;
+    };
+    if (int rc = issue_group(0)) return rc;
+    for (int k = 0;; ++k) {
+      if (int rc = issue_group(k + 1)) return rc;
+      JFC_HIP(hipEventSynchronize(g->cnt_ev[k & 1]));
+      if (g->h_cnt[k & 1] == 0) break;
       if (pass > 1000000) return fail(JFC_ERR_STATE, "search did not settle");
     }
-    long long after = 0;
-    if (int rc = read_ll(total, &after)) return rc;
-    *tiles += after - before + (mode == kRelabel ? g->ntiles : 0);
     return JFC_OK;
   }
 
   // Tag post-process over the current labels; g->tag holds the foreground
   // seed bytes on entry.
-  int tag_pass(int *passes, long long *tiles) {
+  int tag_pass(int *passes) {
     launch_fill_i32(g->d_ctr, 2 * QSLOT, 0, st);
     g->slot = 0;
     launch_tag_init(g->shape, g->tiles(), g->tag, g->d_seed, st);
-    return search(kTag, passes, tiles);
+    return search(kTag, passes);
   }
 
   int global_relabel() {
     hipEvent_t a = tm.mark();
     launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
-    long long tiles = 0;
-    int rc = search(kRelabel, &s.relabel_passes, &tiles);
+    int rc = search(kRelabel, &s.relabel_passes);
     if (rc) return rc;
-    s.relabel_tiles += tiles;  // k_relax left every tile's active flag
     gr_ev.push_back({a, tm.mark()});
     s.global_relabels++;
     return JFC_OK;
@@ -805,10 +818,8 @@ struct Solver {
     hipEvent_t a = tm.mark();
     launch_fill_i32(g->d_flagword, 1, 0, st);
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, st);
-    long long tiles = 0;
-    int rc = search(kCut, &s.cut_passes, &tiles);
+    int rc = search(kCut, &s.cut_passes);
     if (rc) return rc;
-    s.cut_tiles += tiles;
     cut_ev.push_back({a, tm.mark()});
     return read_int(g->d_flagword, found);
   }
@@ -1010,7 +1021,9 @@ struct Solver {
     s.local_iters = 0;
     for (int k = 0; k < 64; ++k) s.local_iters += totals[kIterSlots + k];
     g->last_sweeps = sweep;
-    s.relabel_tiles += 0;
+    // a relabel's first pass visits every tile without a compaction
+    s.relabel_tiles = totals[1] + (long long)g->ntiles * s.global_relabels;
+    s.cut_tiles = totals[2];
     s.push_bytes = totals[0] * 256LL * push_bytes_per_node(g->Kg);
     s.ms_total = Timer::ms(t0, t1);
     for (auto &p : push_ev) s.ms_push += Timer::ms(p.first, p.second);
@@ -1116,8 +1129,7 @@ extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *
   S.tm.pool = &g->ev_pool;
   S.tm.st = 0;
   int passes = 0;
-  long long tiles = 0;
-  if ((rc = S.tag_pass(&passes, &tiles))) return rc;
+  if ((rc = S.tag_pass(&passes))) return rc;
   return get_rows(g, g->inst[inst], tags, g->tag + o, 1, on_device);
 }
 
