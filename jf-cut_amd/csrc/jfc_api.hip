@@ -159,8 +159,26 @@ struct jfc_graph {
   int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
   hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
   hipEvent_t priv_event = nullptr;
-  int *h_cnt = nullptr;                 // pinned: last list counts of two pass groups
+  // Pinned host words the solve's decisions read (asynchronous copies, then
+  // a wait on an event instead of a staged pageable copy).
+  struct HostPin {
+    long long total;  // tile visits so far (sweep checks)
+    int cnt[2];       // last list count of two pass groups (searches)
+    int recent[3];    // active tiles of the latest sweeps
+    int flag;         // certificate word of a cut search
+  } *pin = nullptr;
   hipEvent_t cnt_ev[2] = {nullptr, nullptr};
+  int ensure_pin() {
+    if (pin) return JFC_OK;
+    for (hipEvent_t &ev : cnt_ev)
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+        return fail(JFC_ERR_DEVICE, "event creation failed");
+    if (hipHostMalloc((void **)&pin, sizeof(HostPin), hipHostMallocDefault) != hipSuccess) {
+      pin = nullptr;
+      return fail(JFC_ERR_OOM, "pinned host words");
+    }
+    return JFC_OK;
+  }
   // Captured launch sequence of one sweep, per starting queue slot.
   struct SweepGraph {
     hipGraphExec_t exec = nullptr;
@@ -201,8 +219,8 @@ struct jfc_graph {
       if (ev) (void)hipEventDestroy(ev);
       ev = nullptr;
     }
-    if (h_cnt) (void)hipHostFree(h_cnt);
-    h_cnt = nullptr;
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
@@ -716,11 +734,6 @@ struct Solver {
   }
   void advance() { g->slot ^= 1; }
 
-  int read_int(const int *dptr, int *out) {
-    JFC_HIP(hipMemcpyAsync(out, dptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipStreamSynchronize(st));
-    return JFC_OK;
-  }
   // relabel / cut passes issued between two host checks
   static constexpr int kPassGroup = 4;
 
@@ -742,10 +755,7 @@ struct Solver {
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
-    if (!g->h_cnt) {
-      JFC_HIP(hipHostMalloc((void **)&g->h_cnt, 2 * sizeof(int), hipHostMallocDefault));
-      for (hipEvent_t &ev : g->cnt_ev) JFC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    }
+    if (int rc = g->ensure_pin()) return rc;
     auto flood = [&](const Queue &q, bool full_pass) {
       if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
       else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
@@ -779,7 +789,7 @@ struct Solver {
         *passes += 1;
       }
       // the last compaction's slot is the one before the current
-      JFC_HIP(hipMemcpyAsync(g->h_cnt + (k & 1), g->d_ctr + QSLOT * (g->slot ^ 1), sizeof(int),
+      JFC_HIP(hipMemcpyAsync(&g->pin->cnt[k & 1], g->d_ctr + QSLOT * (g->slot ^ 1), sizeof(int),
                              hipMemcpyDeviceToHost, st));
       JFC_HIP(hipEventRecord(g->cnt_ev[k & 1], st));
       return JFC_OK;
@@ -788,7 +798,7 @@ struct Solver {
     for (int k = 0;; ++k) {
       if (int rc = issue_group(k + 1)) return rc;
       JFC_HIP(hipEventSynchronize(g->cnt_ev[k & 1]));
-      if (g->h_cnt[k & 1] == 0) break;
+      if (g->pin->cnt[k & 1] == 0) break;
       if (pass > 1000000) return fail(JFC_ERR_STATE, "search did not settle");
     }
     return JFC_OK;
@@ -821,7 +831,11 @@ struct Solver {
     int rc = search(kCut, &s.cut_passes);
     if (rc) return rc;
     cut_ev.push_back({a, tm.mark()});
-    return read_int(g->d_flagword, found);
+    JFC_HIP(hipMemcpyAsync(&g->pin->flag, g->d_flagword, sizeof(int), hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipEventRecord(g->cnt_ev[0], st));
+    JFC_HIP(hipEventSynchronize(g->cnt_ev[0]));
+    *found = g->pin->flag;
+    return JFC_OK;
   }
 
   // One push-relabel sweep: per colour class, compact the flagged tiles and
@@ -919,16 +933,12 @@ struct Solver {
     launch_fill_u8(g->d_mat, g->ntiles, 0, st);
     // Fast-Cut reads and writes the residual planes directly
     if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
+    if (int rc0 = g->ensure_pin()) return rc0;
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
+    // the fold's sums are read with the results (no host wait here): an
+    // instance without terminal capacity has no excess, so the solve passes
+    // over it, and the error is reported at the end
     std::vector<long long> red((size_t)nI * RED_SLOTS);
-    JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
-                           hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipStreamSynchronize(st));
-    for (int i = 0; i < nI; ++i)
-      if (red[i * RED_SLOTS + RED_DEF0] == 0 && red[i * RED_SLOTS + RED_SUP0] == 0 &&
-          red[i * RED_SLOTS + RED_CONST] == 0)
-        return fail(JFC_ERR_NO_SEEDS, "instance " + std::to_string(i) +
-                                          " has no terminal capacity (no labels)");
     int rc;
     if (o.initial_relabel) {
       if ((rc = global_relabel())) return rc;
@@ -962,12 +972,14 @@ struct Solver {
           // active-tile counts of the sweeps since the last global relabel
           const int first = std::max(sweep_at_gr, sweep - 2);
           recent.assign(sweep - first, 0);
-          long long total = 0;
-          JFC_HIP(hipMemcpyAsync(recent.data(), g->d_sweep + first, recent.size() * sizeof(int),
+          JFC_HIP(hipMemcpyAsync(g->pin->recent, g->d_sweep + first, recent.size() * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
-          JFC_HIP(hipMemcpyAsync(&total, g->d_total, sizeof(long long),
+          JFC_HIP(hipMemcpyAsync(&g->pin->total, g->d_total, sizeof(long long),
                                  hipMemcpyDeviceToHost, st));
-          JFC_HIP(hipStreamSynchronize(st));
+          JFC_HIP(hipEventRecord(g->cnt_ev[0], st));
+          JFC_HIP(hipEventSynchronize(g->cnt_ev[0]));
+          for (size_t i = 0; i < recent.size(); ++i) recent[i] = g->pin->recent[i];
+          const long long total = g->pin->total;
           const int cnt = recent.back();
           if (cnt == 0) { active = false; break; }
           // relabel when the active set stops shrinking (nodes cut off from
@@ -1013,6 +1025,11 @@ struct Solver {
     long long totals[kTotals];
     JFC_HIP(hipMemcpyAsync(totals, g->d_total, sizeof(totals), hipMemcpyDeviceToHost, st));
     JFC_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nI; ++i)
+      if (red[i * RED_SLOTS + RED_DEF0] == 0 && red[i * RED_SLOTS + RED_SUP0] == 0 &&
+          red[i * RED_SLOTS + RED_CONST] == 0)
+        return fail(JFC_ERR_NO_SEEDS, "instance " + std::to_string(i) +
+                                          " has no terminal capacity (no labels)");
     for (int i = 0; i < nI; ++i)
       g->flows[i] = red[i * RED_SLOTS + RED_CONST] + red[i * RED_SLOTS + RED_DEF0] - def[i];
     s.flow = g->flows[0];
