@@ -167,14 +167,22 @@ struct jfc_graph {
     int recent[3];    // active tiles of the latest sweeps
     int flag;         // certificate word of a cut search
   } *pin = nullptr;
+  // Pinned copies of a solve's results: per-instance sums (RED_SLOTS each),
+  // remaining deficits, then the kTotals counters.
+  long long *pin_res = nullptr;
   hipEvent_t cnt_ev[2] = {nullptr, nullptr};
   int ensure_pin() {
     if (pin) return JFC_OK;
     for (hipEvent_t &ev : cnt_ev)
       if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
         return fail(JFC_ERR_DEVICE, "event creation failed");
-    if (hipHostMalloc((void **)&pin, sizeof(HostPin), hipHostMallocDefault) != hipSuccess) {
+    const size_t nres = inst.size() * (RED_SLOTS + 1) + kTotals;
+    if (hipHostMalloc((void **)&pin, sizeof(HostPin), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&pin_res, nres * sizeof(long long), hipHostMallocDefault) !=
+            hipSuccess) {
+      if (pin) (void)hipHostFree(pin);
       pin = nullptr;
+      pin_res = nullptr;
       return fail(JFC_ERR_OOM, "pinned host words");
     }
     return JFC_OK;
@@ -220,7 +228,9 @@ struct jfc_graph {
       ev = nullptr;
     }
     if (pin) (void)hipHostFree(pin);
+    if (pin_res) (void)hipHostFree(pin_res);
     pin = nullptr;
+    pin_res = nullptr;
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
@@ -1017,14 +1027,15 @@ struct Solver {
     // the last cut search ran on the final excess and left the remaining
     // deficit of every instance in d_def
     hipEvent_t t1 = tm.mark();
-    std::vector<long long> def((size_t)nI);
-    JFC_HIP(hipMemcpyAsync(red.data(), g->d_red, red.size() * sizeof(long long),
+    long long *pred = g->pin_res, *pdef = pred + red.size(), *totals = pdef + nI;
+    JFC_HIP(hipMemcpyAsync(pred, g->d_red, red.size() * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipMemcpyAsync(def.data(), g->d_def, def.size() * sizeof(long long),
+    JFC_HIP(hipMemcpyAsync(pdef, g->d_def, nI * sizeof(long long), hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipMemcpyAsync(totals, g->d_total, kTotals * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
-    long long totals[kTotals];
-    JFC_HIP(hipMemcpyAsync(totals, g->d_total, sizeof(totals), hipMemcpyDeviceToHost, st));
     JFC_HIP(hipStreamSynchronize(st));
+    std::copy(pred, pred + red.size(), red.begin());
+    const std::vector<long long> def(pdef, pdef + nI);
     for (int i = 0; i < nI; ++i)
       if (red[i * RED_SLOTS + RED_DEF0] == 0 && red[i * RED_SLOTS + RED_SUP0] == 0 &&
           red[i * RED_SLOTS + RED_CONST] == 0)
