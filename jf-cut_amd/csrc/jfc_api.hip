@@ -135,6 +135,8 @@ struct jfc_graph {
   int *d_list[2] = {nullptr, nullptr};
   int *d_ctr = nullptr;
   unsigned char *d_flag = nullptr, *d_dirty = nullptr, *d_seed = nullptr;
+  unsigned char *d_defi = nullptr;  // tiles that may hold a deficit node
+  int gr_tag = 1;                   // tag of the last global relabel's heights
   unsigned char *d_mat = nullptr;  // per tile: residual planes written this solve
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
@@ -207,7 +209,7 @@ struct jfc_graph {
   void release_tiles() {
     drop_graphs();
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
-    dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_jf0); dfree(d_jf1);
+    dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_defi); dfree(d_jf0); dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
     dfree(d_lines);
   }
@@ -327,6 +329,7 @@ struct jfc_graph {
         (rc = dalloc(&d_flag, ntiles, "tile flags")) ||
         (rc = dalloc(&d_dirty, ntiles, "tile flags")) ||
         (rc = dalloc(&d_seed, ntiles, "tile seeds")) ||
+        (rc = dalloc(&d_defi, ntiles, "tile seeds")) ||
         (rc = dalloc(&d_jf0, ntiles, "jump flooding")) ||
         (rc = dalloc(&d_jf1, ntiles, "jump flooding")) ||
         (rc = dalloc(&d_tdist, ntiles, "tile distance")) ||
@@ -758,9 +761,11 @@ struct Solver {
     const Tiles T = g->tiles();
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
-    launch_tile_order(g->shape, T, g->d_seed, g->d_jf0, g->d_jf1, g->d_tdist, g->d_hist,
-                      g->nbins, g->d_order, g->max_tile_dim, nullptr, g->lines(),
-                      o.tile_order, st);
+    // relabel seeds: the tiles holding a deficit node (d_defi, kept by the
+    // fold and the push visits); cut and tag seeds: d_seed from their init
+    launch_tile_order(g->shape, T, mode == kRelabel ? g->d_defi : g->d_seed, g->d_jf0, g->d_jf1,
+                      g->d_tdist, g->d_hist, g->nbins, g->d_order, g->max_tile_dim, nullptr,
+                      g->lines(), o.tile_order, st);
     launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
@@ -769,7 +774,8 @@ struct Solver {
     auto flood = [&](const Queue &q, bool full_pass) {
       if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
       else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
-      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, full_pass, st);
+      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, full_pass && g->dim != 3,
+                        g->gr_tag, st);
     };
     if (mode != kRelabel) {
       // tiles without a seed gain labels only through their halo, so the
@@ -823,9 +829,20 @@ struct Solver {
     return search(kTag, passes);
   }
 
+  // Exact relabel (graphcut_init_global_relabel + graphcut_global_relabel,
+  // graphcut_global_relabel.cl:19-124).  In 2D there is no separate init pass
+  // over the nodes: the first tile pass starts every tile from e and marks
+  // the heights it writes with a fresh tag, so a neighbour's stale height is
+  // recognised and replaced by its start value (config 2 relabel 1.03 ->
+  // 0.94 ms).  3D relabels keep the pass, which also makes the tile seeds
+  // exact: 3D tiles have a halo as large as the tile, and without the pass
+  // (halo excess read in the first tile pass, seeds from the push visits'
+  // deficit flags) config 3 took 14.7 ms instead of 13.4.
   int global_relabel() {
     hipEvent_t a = tm.mark();
-    launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, st);
+    g->gr_tag = 3 - g->gr_tag;
+    if (g->dim == 3)
+      launch_gr_init(g->shape, g->tiles(), g->planes(), g->gr_tag, g->d_defi, st);
     int rc = search(kRelabel, &s.relabel_passes);
     if (rc) return rc;
     gr_ev.push_back({a, tm.mark()});
@@ -862,7 +879,7 @@ struct Solver {
                      g->d_sweep_state, g->d_total, s_);
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
-                  o.max_local_iters, g->d_total + kIterSlots, s_);
+                  g->d_defi, o.max_local_iters, g->d_total + kIterSlots, s_);
       if (timed && o.profile) push_ev.push_back({a, tm.mark()});
       advance();
     }
@@ -877,7 +894,7 @@ struct Solver {
                    g->d_total, st);
     hipEvent_t a = o.profile ? tm.mark() : nullptr;
     launch_fc_sweep(g->shape, std::min(g->nwg, g->ntiles), g->tiles(), g->planes(), q,
-                    g->d_flag, st);
+                    g->d_flag, g->d_defi, st);
     if (o.profile) push_ev.push_back({a, tm.mark()});
     advance();
     launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, st);
@@ -944,7 +961,7 @@ struct Solver {
     // Fast-Cut reads and writes the residual planes directly
     if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     if (int rc0 = g->ensure_pin()) return rc0;
-    launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, st);
+    launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi, st);
     // the fold's sums are read with the results (no host wait here): an
     // instance without terminal capacity has no excess, so the solve passes
     // over it, and the error is reported at the end
@@ -1231,9 +1248,9 @@ extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
   if (!g || !flags || which < 0 || which > 2) return fail(JFC_ERR_INVALID, "bad argument");
   const int nI = (int)g->inst.size();
   launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, 0);
-  launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr, 0);
-  unsigned char *src = g->d_flag;
-  if (which == 0) { launch_gr_init(g->shape, g->tiles(), g->planes(), g->d_seed, 0); src = g->d_seed; }
+  launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr,
+              g->d_defi, 0);
+  unsigned char *src = which == 0 ? g->d_defi : g->d_flag;
   if (which == 1) {
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, 0);
     src = g->d_seed;

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_fc_relabel(Tiles T, Planes P, Queue q, 
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     if (gx >= I.nx || gy >= I.ny || gz >= I.nz || ((gx + gy + gz) & 1) != parity) continue;
     const long long v = node_index(I, gx, gy, gz);
-    const int h = P.h[v];
+    const int h = P.h[v] & HMASK;
     if (P.e[v] < 0 || h >= HINF) continue;
     const unsigned m = P.rmask[v];
     const long long nn = (long long)I.nx * I.ny * I.nz;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_fc_relabel(Tiles T, Planes P, Queue q, 
       if (!((m >> d) & 1u)) continue;
       int dx, dy, dz;
       dir_of<S::D3>(d, dx, dy, dz);
-      nh = min(nh, P.h[node_index(I, gx + dx, gy + dy, gz + dz)] + 1);
+      nh = min(nh, (P.h[node_index(I, gx + dx, gy + dy, gz + dz)] & HMASK) + 1);
     }
     if (nh >= hcap) nh = HINF;
     if (nh != h) P.h[v] = nh;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
     for (int k = 0; k < n; ++k) {
       const long long v = v0 + k * step;
       e[k] = e0[k] = P.e[v];
-      h[k] = P.h[v];
+      h[k] = P.h[v] & HMASK;
       cf[k] = cf0[k] = P.cap[F * st + v];
       cb[k] = cb0[k] = P.cap[B * st + v];
     }
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
       const int f = min(cf[n - 1], e[n - 1]);
       if (f > 0) {
         const long long u = v0 + n * step;
-        if (h[n - 1] - P.h[u] == 1) {
+        if (h[n - 1] - (P.h[u] & HMASK) == 1) {
           cf[n - 1] -= f;
           e[n - 1] -= f;
           const int c = P.cap[B * st + u];
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
       const int f = min(cb[0], e[0]);
       if (f > 0) {
         const long long u = v0 - step;
-        if (h[0] - P.h[u] == 1) {
+        if (h[0] - (P.h[u] & HMASK) == 1) {
           cb[0] -= f;
           e[0] -= f;
           const int c = P.cap[F * st + u];
@@ -165,23 +165,30 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
   }
 }
 
-// Active flag of every tile: a node with excess and a finite label.
+// Active flag of every tile (a node with excess and a finite label) and its
+// deficit flag (a node with e < 0, the relabel's ordering seeds).
 template <class S>
-__global__ __launch_bounds__(256) void k_fc_flags(Tiles T, Planes P, unsigned char *flag) {
-  __shared__ int s_any;
+__global__ __launch_bounds__(256) void k_fc_flags(Tiles T, Planes P, unsigned char *flag,
+                                                  unsigned char *defi) {
+  __shared__ int s_any, s_def;
   const Lane<S> L(threadIdx.x);
   for (int t = blockIdx.x; t < T.ntiles; t += gridDim.x) {
-    if (threadIdx.x == 0) s_any = 0;
+    if (threadIdx.x == 0) s_any = s_def = 0;
     __syncthreads();
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
     if (gx < I.nx && gy < I.ny && gz < I.nz) {
       const long long v = node_index(I, gx, gy, gz);
-      if (P.e[v] > 0 && P.h[v] < HINF) s_any = 1;
+      const int e = P.e[v];
+      if (e > 0 && (P.h[v] & HMASK) < HINF) s_any = 1;
+      if (e < 0) s_def = 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0) flag[t] = (unsigned char)s_any;
+    if (threadIdx.x == 0) {
+      flag[t] = (unsigned char)s_any;
+      defi[t] = (unsigned char)s_def;
+    }
     __syncthreads();
   }
 }
@@ -205,7 +212,7 @@ void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStr
 }
 
 void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
-                     unsigned char *flag, hipStream_t st) {
+                     unsigned char *flag, unsigned char *defi, hipStream_t st) {
   const int d3 = shape_is_3d(shape);
   JFC_DISPATCH(shape, ({
     for (int par = 0; par < 2; ++par)
@@ -217,7 +224,7 @@ void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const 
     if (d3)
       for (int par = 0; par < 2; ++par)
         k_fc_push<S, 2><<<nwg, 256, 0, st>>>(T, P, q, par);
-    k_fc_flags<S><<<std::min(T.ntiles, 8192), 256, 0, st>>>(T, P, flag);
+    k_fc_flags<S><<<std::min(T.ntiles, 8192), 256, 0, st>>>(T, P, flag, defi);
   }));
 }
 

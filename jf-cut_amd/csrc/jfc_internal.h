@@ -10,6 +10,18 @@ namespace jfc {
 
 // Height of a node proven unable to reach any deficit node.
 constexpr int HINF = 0x3fffffff;
+// A height word holds the height in bits 0-29 and, in bits 30-31, the tag of
+// the global relabel that wrote it (1 or 2, alternating; 0 from the push
+// kernels and the zero start).  Every reader masks with HMASK.  A relabel's
+// first pass takes a neighbour's word as a distance bound only when it
+// carries the running relabel's tag; any other word is stale, and the
+// neighbour's start value (0 at a deficit, HINF elsewhere) is used instead.
+// The first pass rewrites every word with the tag, so a word left from the
+// relabel before last can never carry the running tag.
+constexpr int HMASK = 0x3fffffff;
+__host__ __device__ inline int tag_word(int h, int tag) {
+  return (int)((unsigned)h | ((unsigned)tag << 30));
+}
 
 // Per-instance reduction slots (int64): folded constant, initial deficit,
 // energy, initial supply.  (The final deficit has an array of its own.)
@@ -37,7 +49,7 @@ struct InstDesc {
 
 struct Planes {
   int *e;            // excess (> 0) / deficit (< 0), folded terminal arcs
-  int *h;            // height; HINF = cannot reach a deficit
+  int *h;            // height word (HMASK bits); HINF = cannot reach a deficit
   int *cap;          // K residual planes, plane d at cap + d * stride
   const int *c0;     // input neighbour planes (same layout)
   const int *cs;     // input source capacities
@@ -91,8 +103,10 @@ void set_last_error(const std::string &msg);
 void launch_fill_i32(int *p, long long n, int v, hipStream_t st);
 void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t st);
 
+// active (or null): tiles holding an excess node; defi: tiles holding a
+// deficit node (the relabel's jump-flooding seeds).  Both zeroed here.
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, hipStream_t st);
+                 unsigned char *active, unsigned char *defi, hipStream_t st);
 void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
@@ -102,8 +116,6 @@ void launch_seeds(int *cs, int *ct, const unsigned char *fg,
 void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
                          hipStream_t st);
 
-void launch_gr_init(int shape, const Tiles &T, const Planes &P,
-                    unsigned char *seed, hipStream_t st);
 // Also sums the remaining deficit max(-e, 0) per instance into def[inst]
 // (def must be zeroed by the caller).
 void launch_cut_init(int shape, const Tiles &T, const Planes &P,
@@ -134,18 +146,29 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
                        hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in
-// this solve (cleared when a solve starts).  iter_total: 64 int64 counters
-// of local iterations (or null).
+// this solve (cleared when a solve starts).  defi: a visit rewrites its
+// tile's deficit flag (pushes never create deficits, so unvisited tiles'
+// flags stay a superset).  iter_total: 64 int64 counters of local
+// iterations (or null).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 int max_iters, long long *iter_total, hipStream_t st);
+                 unsigned char *defi, int max_iters, long long *iter_total,
+                 hipStream_t st);
 // Fast-Cut comparator (jfc_fastcut.hip)
 void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);
 void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
-                     unsigned char *flag, hipStream_t st);
+                     unsigned char *flag, unsigned char *defi, hipStream_t st);
+// Relabel start values written over all nodes with `tag`, and the exact
+// deficit flags of the tiles (3D relabels).
+void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
+                    hipStream_t st);
+// One relabel pass.  The first pass (first_pass, every tile) starts each
+// tile from e (0 at deficits, HINF elsewhere) and writes every height word
+// with `tag`; later passes, and every pass after launch_gr_init, relax the
+// queued tiles from their stored heights.
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  bool full_pass, hipStream_t st);
+                  bool first_pass, int tag, hipStream_t st);
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st);

@@ -189,6 +189,17 @@ __device__ __forceinline__ int block_any(int pred, int *s_w, int &parity) {
   return buf[0] | buf[1] | buf[2] | buf[3];
 }
 
+// block_any for two bits at once: bit b of the result is set when some lane
+// has bit b of `bits` set (one ballot per bit, one barrier).
+__device__ __forceinline__ int block_any2(int bits, int *s_w, int &parity) {
+  const int w = (__ballot(bits & 1) != 0 ? 1 : 0) | (__ballot(bits & 2) != 0 ? 2 : 0);
+  int *buf = s_w + 4 * parity;
+  parity ^= 1;
+  if (lane_id() == 0) buf[threadIdx.x >> 6] = w;
+  __syncthreads();
+  return buf[0] | buf[1] | buf[2] | buf[3];
+}
+
 // Node of lane L in tile t; inst = -1 past the end of the chunk.
 struct TileNode {
   int inst;
@@ -355,10 +366,13 @@ static int row_grid(int nrows) { return nrows < 2048 ? (nrows > 0 ? nrows : 1) :
 // deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
 // With `active` set it also starts the labelling at zero heights (the
 // reference clears h, QGCWidget.cpp:2400) and flags the tiles holding excess
-// (`active` zeroed by the launcher).
+// (`active` zeroed by the launcher).  It always flags the tiles holding a
+// deficit node in `defi`, the seeds of a global relabel's tile ordering (the
+// seed flags of graphcut_init_global_relabel, graphcut_global_relabel.cl:
+// 19-48, kept current by the push visits instead of a pass over e).
 template <class S>
 __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
-                                              unsigned char *active) {
+                                              unsigned char *active, unsigned char *defi) {
   __shared__ long long buf[4];
   long long *dst[3] = {red + RED_CONST, red + RED_DEF0, red + RED_SUP0};
   const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
@@ -387,15 +401,17 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
         e.z = a[u].z - b[u].z; e.w = a[u].w - b[u].w;
         store4(P.e, v, g[u].n, e);
         if (active) store4(P.h, v, g[u].n, make_int4(0, 0, 0, 0));
-        unsigned bits = 0;
+        unsigned bits = 0, dbits = 0;
         for (int i = 0; i < g[u].n; ++i) {
           const int s = comp(a[u], i), t = comp(b[u], i);
           sums.a[0] += min(s, t);
           sums.a[1] += max(t - s, 0);
           sums.a[2] += max(s - t, 0);
           bits |= (unsigned)(s > t) << i;
+          dbits |= (unsigned)(s < t) << i;
         }
         if (active) flag_group<S>(active, R.tile0, g[u], bits);
+        flag_group<S>(defi, R.tile0, g[u], dbits);
       }
     }
   }
@@ -403,9 +419,11 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
 }
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, hipStream_t st) {
+                 unsigned char *active, unsigned char *defi, hipStream_t st) {
   if (active) launch_fill_u8(active, T.ntiles, 0, st);
-  JFC_DISPATCH(shape, (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active)));
+  launch_fill_u8(defi, T.ntiles, 0, st);
+  JFC_DISPATCH(shape,
+               (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active, defi)));
 }
 
 // graphcut_init_cut (graphcut_compute_cut.cl:19-49): label = 1 where
@@ -460,11 +478,15 @@ void launch_cut_init(int shape, const Tiles &T, const Planes &P,
                (k_cut_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, seed, def)));
 }
 
-// graphcut_init_global_relabel (graphcut_global_relabel.cl:19-48): h = 0 at
-// deficit nodes, unvisited elsewhere; tiles holding a deficit node seed the
-// jump flooding (`seed` zeroed by the launcher).
+// graphcut_init_global_relabel (graphcut_global_relabel.cl:19-48) as a pass
+// over the nodes: h = 0 at deficit nodes, HINF elsewhere, written with the
+// relabel's tag, and the exact deficit flags of the tiles (`defi`, zeroed by
+// the launcher).  Used before the first tile pass of a 3D relabel; in 2D the
+// first tile pass starts from e itself.
 template <class S>
-__global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned char *seed) {
+__global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
+                                                 unsigned char *defi) {
+  const int zero = tag_word(0, tag), inf = tag_word(HINF, tag);
   for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
     const RowRef<S> R = row_ref<S>(T, r);
     const int ng = row_groups(R.v0, R.nx);
@@ -481,21 +503,21 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, unsigned cha
       for (int u = 0; u < RS_U; ++u) {
         if (g[u].n == 0) continue;
         int4 h;
-        h.x = e[u].x < 0 ? 0 : HINF; h.y = e[u].y < 0 ? 0 : HINF;
-        h.z = e[u].z < 0 ? 0 : HINF; h.w = e[u].w < 0 ? 0 : HINF;
+        h.x = e[u].x < 0 ? zero : inf; h.y = e[u].y < 0 ? zero : inf;
+        h.z = e[u].z < 0 ? zero : inf; h.w = e[u].w < 0 ? zero : inf;
         store4(P.h, R.v0 + g[u].x, g[u].n, h);
         unsigned bits = 0;
         for (int i = 0; i < g[u].n; ++i) bits |= (unsigned)(comp(e[u], i) < 0) << i;
-        flag_group<S>(seed, R.tile0, g[u], bits);
+        flag_group<S>(defi, R.tile0, g[u], bits);
       }
     }
   }
 }
 
-void launch_gr_init(int shape, const Tiles &T, const Planes &P,
-                    unsigned char *seed, hipStream_t st) {
-  launch_fill_u8(seed, T.ntiles, 0, st);
-  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, seed)));
+void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
+                    hipStream_t st) {
+  launch_fill_u8(defi, T.ntiles, 0, st);
+  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, tag, defi)));
 }
 
 // Residual-arc bits of the input planes (see the mask comment below).
@@ -1131,7 +1153,8 @@ template <class S>
 // 5 waves per SIMD: the register budget that keeps the load phase unspilled.
 __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
-                                              int max_iters, long long *iter_total) {
+                                              unsigned char *defi, int max_iters,
+                                              long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -1175,10 +1198,10 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
 #pragma unroll
     for (int c = 0; c < S::NCELL; ++c) {
       hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
-      hh[c] = P.h[hc[c].g];
+      hh[c] = P.h[hc[c].g] & HMASK;
       hm[c] = P.rmask[hc[c].g];
     }
-    int e0 = P.e[gv], h0 = P.h[gv];
+    int e0 = P.e[gv], h0 = P.h[gv] & HMASK;
     unsigned rm = 0;
     int c0r[S::KMAX];
     if (!fresh_tile) {
@@ -1262,7 +1285,7 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
     my_iters += it + (any ? 0 : 1);
 
     // ---- write back what changed (stores only)
-    int act = 0;
+    int act = 0, dfc = 0;
     if (valid) {
       const int e1 = s_e[L.me], h1 = s_h[L.me];
       if (e1 != e0) P.e[gv] = e1;
@@ -1280,6 +1303,7 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
         }
       if (m1 != (fresh_tile ? rm : m0)) P.rmask[gv] = m1;
       act = e1 > 0 && h1 < HINF;
+      dfc = e1 < 0;
     }
     // ---- flush flow that crossed into the halo: excess and the reverse
     // residual of each crossing arc (graphcut_push_relabel.cl:184-195).
@@ -1323,7 +1347,12 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
         if (dc[d] != 0 && !((fresh >> d) & 1u)) atomicAdd(&P.cap[d * st + g], dc[d]);
       if (flip) atomicXor(&P.rmask[g], flip);
     }
-    if (block_any(act, s_w, par) && tid == 0) flag[t] = 1;
+    // the tile's active flag and deficit flag, one barrier for both
+    const int ad = block_any2(act | (dfc << 1), s_w, par);
+    if (tid == 0) {
+      if (ad & 1) flag[t] = 1;
+      defi[t] = (unsigned char)(ad >> 1);
+    }
     if (fresh_tile && tid == 0) mat[t] = 1;
   }
   // one atomic per workgroup, spread over 64 counters
@@ -1333,9 +1362,10 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 int max_iters, long long *iter_total, hipStream_t st) {
+                 unsigned char *defi, int max_iters, long long *iter_total,
+                 hipStream_t st) {
   JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
-                          T, P, q, flag, mat, max_iters, iter_total)));
+                          T, P, q, flag, mat, defi, max_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
@@ -1344,10 +1374,11 @@ void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
 // LDS until the tile is stable; tiles whose halo changed are revisited.  The
 // fixpoint over all tiles is the exact breadth-first distance to the nearest
 // deficit node (graphcut_global_relabel.cl:50-124).
-template <class S>
+template <class S, bool FIRST>
 __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty,
-                                               unsigned char *active) {
+                                               unsigned char *active, int tag) {
+  constexpr bool first = FIRST;
   __shared__ int s_h[S::PADN];
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
@@ -1367,18 +1398,23 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     // halo cells read a valid dummy address)
     const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
     HaloCell hc[S::NCELL];
-    int hh[S::NCELL];
+    int hh[S::NCELL], he[S::NCELL];
 #pragma unroll
     for (int c = 0; c < S::NCELL; ++c) {
       hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
       // device-coherent read: a neighbour tile settled earlier in this same
       // pass (possibly on another XCD) is seen at once, so one pass carries
-      // the distance front across many tiles in list order.  Any value read
-      // is an upper bound of the distance, so stale is safe.
+      // the distance front across many tiles in list order.  Any tagged
+      // value read is an upper bound of the distance, so stale is safe.
       hh[c] = __hip_atomic_load(&P.h[hc[c].g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      he[c] = first ? P.e[hc[c].g] : 0;
     }
-    int h0 = P.h[gv], e0 = P.e[gv];
+    // the first pass starts from e (the relabel's initial labelling) and
+    // needs no stored height
+    int e0 = P.e[gv];
+    int h0 = first ? 0 : P.h[gv] & HMASK;
     unsigned mask = P.rmask[gv] & ((1u << K) - 1u);
+    if (first) h0 = e0 < 0 ? 0 : HINF;
     if (!valid) {
       h0 = HINF;
       e0 = 0;
@@ -1386,8 +1422,16 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     }
     s_h[L.me] = h0;
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c)
-      if (hc[c].halo) s_h[tid + c * S::NT] = hc[c].live ? hh[c] : HINF;
+    for (int c = 0; c < S::NCELL; ++c) {
+      if (!hc[c].halo) continue;
+      int v = HINF;
+      if (hc[c].live) {
+        // an untagged word is left from the pushes or the relabel before:
+        // the neighbour still holds its start value
+        v = ((unsigned)hh[c] >> 30) == (unsigned)tag ? hh[c] & HMASK : (he[c] < 0 ? 0 : HINF);
+      }
+      s_h[tid + c * S::NT] = v;
+    }
     __syncthreads();
     int any;
     do {
@@ -1407,8 +1451,12 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     } while (any);
     if (valid) {
       const int h1 = s_h[L.me];
+      // the first pass writes every word (with the tag); later passes the
+      // changed ones
+      if (first || h1 != h0)
+        __hip_atomic_store(&P.h[gv], tag_word(h1, tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       if (h1 != h0) {
-        __hip_atomic_store(&P.h[gv], h1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d) {
           if (d >= K) break;
@@ -1483,9 +1531,11 @@ struct Ring {
   }
 };
 
-template <class S>
+template <class S, bool FIRST>
 __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
-                                                unsigned char *dirty, unsigned char *active) {
+                                                unsigned char *dirty, unsigned char *active,
+                                                int tag) {
+  constexpr bool first = FIRST;
   constexpr int NJ = S::NT / 64;
   constexpr int NR = (Ring<S>::N + 63) / 64;
   __shared__ int s_h[S::PADN];
@@ -1498,8 +1548,9 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
-    // loads: ring heights (device-coherent, as in k_relax) and the lane's nodes
-    int hh[NR];
+    // loads: ring heights (device-coherent and tagged, as in k_relax) and
+    // the lane's nodes
+    int hh[NR], he[NR];
     bool hlive[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
@@ -1508,6 +1559,7 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
           halo_cell<S>(I, r.ox, r.oy, r.oz, k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN, K);
       hlive[i] = hc.live;
       hh[i] = __hip_atomic_load(&P.h[hc.g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      he[i] = first ? P.e[hc.g] : 0;
     }
     int h0[NJ];
     unsigned mk[NJ];
@@ -1518,9 +1570,11 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
       const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
       const bool valid = gx < I.nx && gy < I.ny && gz < I.nz;
       const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
-      h0[j] = P.h[gv];
+      const int e0 = P.e[gv];
+      h0[j] = first ? 0 : P.h[gv] & HMASK;
       mk[j] = P.rmask[gv] & ((1u << K) - 1u);
-      pos[j] = P.e[gv] > 0;
+      pos[j] = e0 > 0;
+      if (first) h0[j] = e0 < 0 ? 0 : HINF;
       if (!valid) {
         h0[j] = HINF;
         mk[j] = 0;
@@ -1532,7 +1586,11 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const int k = lane + 64 * i;
-      if (k < Ring<S>::N) s_h[Ring<S>::pos(k)] = hlive[i] ? hh[i] : HINF;
+      if (k >= Ring<S>::N) continue;
+      int v = HINF;
+      if (hlive[i])
+        v = ((unsigned)hh[i] >> 30) == (unsigned)tag ? hh[i] & HMASK : (he[i] < 0 ? 0 : HINF);
+      s_h[Ring<S>::pos(k)] = v;
     }
     wave_lds_sync();
     bool any;
@@ -1563,9 +1621,10 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
       if (!(gx < I.nx && gy < I.ny && gz < I.nz)) continue;
       const int h1 = s_h[L.me];
       act |= pos[j] && h1 < HINF;
+      if (first || h1 != h0[j])
+        __hip_atomic_store(&P.h[node_index(I, gx, gy, gz)], tag_word(h1, tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       if (h1 == h0[j]) continue;
-      __hip_atomic_store(&P.h[node_index(I, gx, gy, gz)], h1, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) {
         if (d >= K) break;
@@ -1590,15 +1649,24 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
 // relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  bool full_pass, hipStream_t st) {
-  (void)full_pass;
+                  bool first_pass, int tag, hipStream_t st) {
   if (shape_is_3d(shape)) {
-    JFC_DISPATCH(shape, (k_relax<S><<<resident_grid<&k_relax<S>>(nwg), 256, 0, st>>>(
-                            T, P, q, dirty, active)));
+    if (first_pass) {
+      JFC_DISPATCH(shape, (k_relax<S, true><<<resident_grid<&k_relax<S, true>>(nwg), 256, 0, st>>>(
+                              T, P, q, dirty, active, tag)));
+    } else {
+      JFC_DISPATCH(shape, (k_relax<S, false><<<resident_grid<&k_relax<S, false>>(nwg), 256, 0,
+                                                st>>>(T, P, q, dirty, active, tag)));
+    }
     return;
   }
-  JFC_DISPATCH(shape, (k_relax_w<S><<<resident_grid_b<&k_relax_w<S>, 64>(4 * nwg), 64, 0, st>>>(
-                          T, P, q, dirty, active)));
+  if (first_pass) {
+    JFC_DISPATCH(shape, (k_relax_w<S, true><<<resident_grid_b<&k_relax_w<S, true>, 64>(4 * nwg),
+                                              64, 0, st>>>(T, P, q, dirty, active, tag)));
+  } else {
+    JFC_DISPATCH(shape, (k_relax_w<S, false><<<resident_grid_b<&k_relax_w<S, false>, 64>(4 * nwg),
+                                               64, 0, st>>>(T, P, q, dirty, active, tag)));
+  }
 }
 
 // ---- canonical cut: residual reachability from excess nodes -------------------------
