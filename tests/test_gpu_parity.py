@@ -134,6 +134,35 @@ def test_relabel_serpentine(gpu, shape):
             g.close()
 
 
+@pytest.mark.parametrize("conn,shape", [(4, (150, 97, 1)), (8, (133, 71, 1)), (6, (40, 33, 17))])
+def test_repeated_relabels_gpu(gpu, conn, shape):
+    """Many global relabels on one graph, across solves: a relabel after
+    every sweep (tiny relabel budget), with and without one before the first
+    sweep.  Relabels alternate the tag of the heights they write and a solve
+    starts from the previous solve's words, so a stale word taken for a
+    current one would show as a wrong flow or cut here."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(conn + nx)
+    cs, ct, caps = random_grid(rng, nx, ny, nz, conn)
+    f, l, ok = ol.maxflow(nx, ny, nz, conn, cs, ct, caps)
+    g = gpu.GridGraph([(nx, ny, nz, conn)])
+    g.set_terminal_caps(cs, ct)
+    for d in range(conn):
+        g.set_neighbor_caps(d, caps[d])
+    relabels = 0
+    for k, opts in enumerate([dict(relabel_work=0.01), dict(relabel_work=0.01, initial_relabel=1),
+                              dict(), dict(initial_relabel=1), dict(relabel_work=0.01, check_every=2),
+                              dict(relabel_work=0.01, initial_relabel=1)]):
+        say("case repeated relabels", conn, shape, k, opts)
+        st = g.solve(**opts)
+        relabels += st.global_relabels
+        assert st.converged == 1
+        assert g.flow() == f
+        assert np.array_equal(g.labels(), l)
+    assert relabels >= 8
+    g.close()
+
+
 def test_sweep_graph_replays(gpu):
     """The captured per-sweep hipGraph (default), eager launches and profiled
     launches give identical flows and labels, over many replays on one graph,
