@@ -75,7 +75,9 @@ typedef struct {
   void *stream;           /* hipStream_t to issue work on (NULL = default stream) */
   int initial_relabel;    /* 1 = exact global relabel before the first sweep; 0 =
                              start from zero heights (the reference's default
-                             "no global relabel", QGCW:2394-2401) */
+                             "no global relabel", QGCW:2394-2401); -1 (default)
+                             = the choice jfc_autotune made for the graph's
+                             size class, else 0 */
   double stall_ratio;     /* run a global relabel when the active-tile count of a
                              sweep stays within [r, 1/r] of the previous sweep's
                              (the active set has stopped changing; 0 = off) */
@@ -239,11 +241,12 @@ int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
  * the tile size from the UI, QGCW:826-835) ---------------------------------- */
 
 /* Time `reps` solves of the graph (after one warm-up) with every tile shape of
- * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2) and, unless
- * opt->max_local_iters fixes it, every local-iteration cap of {2, 3, 4}, under
- * the other options of opt (may be NULL), and keep the fastest pair as what
- * the graph's solves use when opt->tile_shape is JFC_TILE_DEFAULT and
- * opt->max_local_iters is 0.  The choice is also stored for the graph's size
+ * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2), crossed with
+ * every local-iteration cap of {2, 3, 4} unless opt->max_local_iters fixes
+ * it and with initial_relabel 0 and 1 unless opt->initial_relabel fixes it,
+ * under the other options of opt (may be NULL), and keep the fastest choice
+ * as what the graph's solves use when opt->tile_shape is JFC_TILE_DEFAULT,
+ * opt->max_local_iters is 0 and opt->initial_relabel is -1.  The choice is also stored for the graph's size
  * class (dimensionality, connectivities, instance count and mean instance
  * size, each rounded to a power of two): graphs of the class created later in
  * the process start with it.  Results are unchanged by either choice.
@@ -255,6 +258,9 @@ int jfc_get_tile_shape(const jfc_graph *g, int *shape);
 
 /* The local-iteration cap a solve with max_local_iters = 0 uses. */
 int jfc_get_local_iters(const jfc_graph *g, int *max_local_iters);
+
+/* The initial_relabel value (0 or 1) a solve with initial_relabel = -1 uses. */
+int jfc_get_initial_relabel(const jfc_graph *g, int *initial_relabel);
 
 /* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
 

@@ -90,6 +90,7 @@ int shape_from_option(int opt, int dim) {
 struct Tuned {
   int shape = 0;
   int mli = 0;
+  int ir = -1;  // initial relabel (-1: not tuned)
 };
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tune;
@@ -112,6 +113,7 @@ struct jfc_graph {
   int dim = 2;
   int tuned = 0;           // JFC_TILE_* chosen by jfc_autotune (0 = none)
   int tuned_mli = 0;       // local-iteration cap chosen by jfc_autotune (0 = none)
+  int tuned_ir = -1;       // initial relabel chosen by jfc_autotune (-1 = none)
   std::string size_class;  // key of the process-wide autotuning table
   int Kg = 4;  // planes stored (max K over instances)
   int shape = 0;
@@ -385,7 +387,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->tile_shape = JFC_TILE_DEFAULT;
   o->profile = 0;
   o->stream = nullptr;
-  o->initial_relabel = 0;
+  o->initial_relabel = -1;
   o->stall_ratio = 0.99;
   o->graph_sweeps = 1;
   o->tile_order = 0;
@@ -493,6 +495,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
     if (it != g_tune.end()) {
       g->tuned = it->second.shape;
       g->tuned_mli = it->second.mli;
+      g->tuned_ir = it->second.ir;
     }
   }
   if ((rc = g->set_shape(default_shape(g->dim)))) { delete g; return rc; }
@@ -1092,6 +1095,8 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.check_every <= 0) S.o.check_every = 1;
   // 0: the cap jfc_autotune chose for the graph's size class, else 2
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
+  // -1: the choice jfc_autotune made for the size class, else none
+  if (S.o.initial_relabel < 0) S.o.initial_relabel = g->tuned_ir > 0 ? 1 : 0;
   if (S.o.tile_order != 0 && S.o.tile_order != 1)
     return fail(JFC_ERR_INVALID, "tile_order must be 0 (distance transform) or 1 (jump flooding)");
   if (S.o.method != JFC_METHOD_JFCUT && S.o.method != JFC_METHOD_FASTCUT)
@@ -1277,32 +1282,41 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   const int mli_fixed = o.max_local_iters > 0 ? o.max_local_iters : 0;
   const int mcand[3] = {2, 3, 4};
   const int nm = mli_fixed ? 1 : 3;
-  int best = 0, best_mli = 0;
+  // so is the relabel before the first sweep: it pays where zero heights
+  // let flow slosh for sweeps (config 3: 13.4 -> 10.2 ms) and costs where
+  // the push sweeps settle most of the flow first (config 2: 4.8 -> 8.5 ms)
+  const int ir_fixed = o.initial_relabel >= 0 ? o.initial_relabel : -1;
+  const int ni = ir_fixed >= 0 ? 1 : 2;
+  int best = 0, best_mli = 0, best_ir = 0;
   double best_ms = 0.0;
   for (int c = 0; c < nc; ++c)
-    for (int m = 0; m < nm; ++m) {
-      o.tile_shape = cand[c];
-      o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
-      std::vector<double> ms;
-      for (int r = 0; r <= reps; ++r) {
-        jfc_stats st;
-        const int rc = jfc_solve(g, &o, &st);
-        if (rc < 0) return rc;
-        if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+    for (int m = 0; m < nm; ++m)
+      for (int i = 0; i < ni; ++i) {
+        o.tile_shape = cand[c];
+        o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
+        o.initial_relabel = ir_fixed >= 0 ? ir_fixed : i;
+        std::vector<double> ms;
+        for (int r = 0; r <= reps; ++r) {
+          jfc_stats st;
+          const int rc = jfc_solve(g, &o, &st);
+          if (rc < 0) return rc;
+          if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (!best || med < best_ms) {
+          best = cand[c];
+          best_mli = o.max_local_iters;
+          best_ir = o.initial_relabel;
+          best_ms = med;
+        }
       }
-      std::sort(ms.begin(), ms.end());
-      const double med = ms[ms.size() / 2];
-      if (!best || med < best_ms) {
-        best = cand[c];
-        best_mli = o.max_local_iters;
-        best_ms = med;
-      }
-    }
   g->tuned = best;
   g->tuned_mli = mli_fixed ? 0 : best_mli;
+  g->tuned_ir = ir_fixed >= 0 ? -1 : best_ir;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[g->size_class] = Tuned{best, g->tuned_mli};
+    g_tune[g->size_class] = Tuned{best, g->tuned_mli, g->tuned_ir};
   }
   if (chosen) *chosen = best;
   return JFC_OK;
@@ -1317,6 +1331,12 @@ extern "C" int jfc_get_tile_shape(const jfc_graph *g, int *shape) {
 extern "C" int jfc_get_local_iters(const jfc_graph *g, int *max_local_iters) {
   if (!g || !max_local_iters) return fail(JFC_ERR_INVALID, "null argument");
   *max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_initial_relabel(const jfc_graph *g, int *initial_relabel) {
+  if (!g || !initial_relabel) return fail(JFC_ERR_INVALID, "null argument");
+  *initial_relabel = g->tuned_ir > 0 ? 1 : 0;
   return JFC_OK;
 }
 

@@ -99,7 +99,7 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
-           "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_default_widget_model_params",
+           "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_get_initial_relabel", "jfc_default_widget_model_params",
            "jfc_model_widget"]
 
 
@@ -157,6 +157,7 @@ def lib():
         "jfc_autotune": (I, [P, ctypes.POINTER(Options), I, ctypes.POINTER(I)]),
         "jfc_get_tile_shape": (I, [P, ctypes.POINTER(I)]),
         "jfc_get_local_iters": (I, [P, ctypes.POINTER(I)]),
+        "jfc_get_initial_relabel": (I, [P, ctypes.POINTER(I)]),
         "jfc_default_widget_model_params": (None, [ctypes.POINTER(WidgetModelParams)]),
         "jfc_model_widget": (I, [P, I, P, P, P, I, ctypes.POINTER(WidgetModelParams),
                                  ctypes.POINTER(WidgetModelStats)]),
@@ -381,6 +382,12 @@ class GridGraph:
         """Local-iteration cap default solves (max_local_iters=0) use."""
         ch = ctypes.c_int()
         _check(lib().jfc_get_local_iters(self._h, ctypes.byref(ch)))
+        return ch.value
+
+    def initial_relabel(self):
+        """initial_relabel value (0/1) default solves (initial_relabel=-1) use."""
+        ch = ctypes.c_int()
+        _check(lib().jfc_get_initial_relabel(self._h, ctypes.byref(ch)))
         return ch.value
 
     def size(self):

@@ -267,10 +267,10 @@ def test_autotune_gpu(gpu):
 
     g = build()
     best = g.autotune(reps=1)
-    mli = g.local_iters()
-    say("case autotune", best, mli)
+    mli, ir = g.local_iters(), g.initial_relabel()
+    say("case autotune", best, mli, ir)
     assert best in ("32x8", "64x4", "16x16") and g.tile_shape() == best
-    assert mli in (2, 3, 4)
+    assert mli in (2, 3, 4) and ir in (0, 1)
     st = g.solve()
     assert st.converged == 1
     for i, (cs, ct, caps) in enumerate(data):
@@ -278,10 +278,12 @@ def test_autotune_gpu(gpu):
         assert g.flow(i) == f and np.array_equal(g.labels(i), l), i
     g.close()
     g2 = build()
-    assert g2.tile_shape() == best and g2.local_iters() == mli
-    # a fixed cap: only the shape is tuned, and the class keeps no cap
-    best2 = g2.autotune(reps=1, max_local_iters=1)
+    assert g2.tile_shape() == best and g2.local_iters() == mli and g2.initial_relabel() == ir
+    # a fixed cap and relabel choice: only the shape is tuned, and the class
+    # keeps neither
+    best2 = g2.autotune(reps=1, max_local_iters=1, initial_relabel=1)
     assert best2 in ("32x8", "64x4", "16x16") and g2.local_iters() == 2
+    assert g2.initial_relabel() == 0
     g2.close()
 
 
