@@ -1136,6 +1136,57 @@ static int resident_grid(int want) {
 // inputs and copied to rmask when a solve starts; k_push reads fresh planes
 // from c0 and writes them to cap, so no separate reset copy of the residual
 // planes is needed.
+// Cells of the padded tile a connectivity reads outside the tile: the sides
+// and corners in 2D, the faces in 3D.  (The edges and corners of a 3D padded
+// tile are never read: 6-connected arcs only cross faces.)
+template <class S>
+struct Ring {
+  static constexpr int N = S::D3 ? 2 * (S::TW * S::TH + S::TW * S::TD + S::TH * S::TD)
+                                 : 2 * S::PW + 2 * S::TH;
+  static __device__ __forceinline__ int pos(int k) {
+    if constexpr (!S::D3) {
+      if (k < S::PW) return k;
+      k -= S::PW;
+      if (k < S::PW) return (S::PH - 1) * S::PW + k;
+      k -= S::PW;
+      if (k < S::TH) return (k + 1) * S::PW;
+      k -= S::TH;
+      return (k + 1) * S::PW + S::PW - 1;
+    } else {
+      constexpr int XY = S::TW * S::TH, XZ = S::TW * S::TD, YZ = S::TH * S::TD;
+      int x, y, z;
+      if (k < 2 * XY) {
+        const int q = k % XY;
+        x = q % S::TW + 1;
+        y = q / S::TW + 1;
+        z = k < XY ? 0 : S::PD - 1;
+      } else if (k < 2 * XY + 2 * XZ) {
+        k -= 2 * XY;
+        const int q = k % XZ;
+        x = q % S::TW + 1;
+        z = q / S::TW + 1;
+        y = k < XZ ? 0 : S::PH - 1;
+      } else {
+        k -= 2 * XY + 2 * XZ;
+        const int q = k % YZ;
+        y = q % S::TH + 1;
+        z = q / S::TH + 1;
+        x = k < YZ ? 0 : S::PW - 1;
+      }
+      return x + S::PW * (y + S::PH * z);
+    }
+  }
+};
+
+// The ring split over the four waves of a 256-lane workgroup: wave w owns the
+// kRingQ consecutive cells from w * kRingQ (consecutive cells are neighbours
+// along a row or face, so a wave's loads stay on few lines), its lanes
+// taking kRingPerLane of them each.
+template <class S>
+constexpr int kRingQ = (Ring<S>::N + 3) / 4;
+template <class S>
+constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
+
 // ---- block-wise push-relabel -------------------------------------------------------
 //
 // One workgroup discharges one tile at a time: e, h and the K residual planes
@@ -1150,8 +1201,10 @@ static int resident_grid(int want) {
 // whole iteration pushes nothing (graphcut_push_relabel.cl:77-182) or after
 // max_iters iterations.
 template <class S>
-// 5 waves per SIMD: the register budget that keeps the load phase unspilled.
-__global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
+// 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
+// without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
+// 8, where it spills).
+__global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
                                               unsigned char *defi, int max_iters,
                                               long long *iter_total) {
@@ -1192,12 +1245,20 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
     long long ca[S::KMAX];
 #pragma unroll
     for (int d = 0; d < S::KMAX; ++d) ca[d] = (valid && d < K) ? d * st + gv : 0;
-    HaloCell hc[S::NCELL];
-    int hh[S::NCELL];
-    unsigned hm[S::NCELL];
+    // halo: the ring cells (sides and corners in 2D, faces in 3D), split
+    // evenly over the four waves so that each wave loads, clears and flushes
+    // the same number of cells; the workgroup barrier after the flush then
+    // waits for no straggling wave
+    HaloCell hc[kRingPerLane<S>];
+    int hp[kRingPerLane<S>];
+    int hh[kRingPerLane<S>];
+    unsigned hm[kRingPerLane<S>];
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
-      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
+      const int j = c * 64 + (tid & 63);
+      const int k = (tid >> 6) * kRingQ<S> + j;
+      hp[c] = j < kRingQ<S> && k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN;
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, hp[c], K);
       hh[c] = P.h[hc[c].g] & HMASK;
       hm[c] = P.rmask[hc[c].g];
     }
@@ -1226,9 +1287,9 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
 #pragma unroll
     for (int d = 0; d < S::KMAX; ++d) s_c[d][L.me] = c0r[d];
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
       if (!hc[c].halo) continue;
-      const int p = tid + c * S::NT;
+      const int p = hp[c];
       s_h[p] = hc[c].live ? hh[c] : HINF;
       s_e[p] = 0;
 #pragma unroll
@@ -1313,9 +1374,9 @@ __global__ __launch_bounds__(256, 5) void k_push(Tiles T, Planes P, Queue q,
     // hence atomics.  A plane that is still fresh (its value lives in c0) is
     // written as c0 + dc; the others take a fire-and-forget atomic add.
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
       if (!hc[c].live) continue;
-      const int p = tid + c * S::NT;
+      const int p = hp[c];
       const long long g = hc[c].g;
       const int f = s_e[p];
       if (f != 0) {
@@ -1490,46 +1551,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Cells of the padded tile a connectivity reads outside the tile: the sides
-// and corners in 2D, the faces in 3D.
-template <class S>
-struct Ring {
-  static constexpr int N = S::D3 ? 2 * (S::TW * S::TH + S::TW * S::TD + S::TH * S::TD)
-                                 : 2 * S::PW + 2 * S::TH;
-  static __device__ __forceinline__ int pos(int k) {
-    if constexpr (!S::D3) {
-      if (k < S::PW) return k;
-      k -= S::PW;
-      if (k < S::PW) return (S::PH - 1) * S::PW + k;
-      k -= S::PW;
-      if (k < S::TH) return (k + 1) * S::PW;
-      k -= S::TH;
-      return (k + 1) * S::PW + S::PW - 1;
-    } else {
-      constexpr int XY = S::TW * S::TH, XZ = S::TW * S::TD, YZ = S::TH * S::TD;
-      int x, y, z;
-      if (k < 2 * XY) {
-        const int q = k % XY;
-        x = q % S::TW + 1;
-        y = q / S::TW + 1;
-        z = k < XY ? 0 : S::PD - 1;
-      } else if (k < 2 * XY + 2 * XZ) {
-        k -= 2 * XY;
-        const int q = k % XZ;
-        x = q % S::TW + 1;
-        z = q / S::TW + 1;
-        y = k < XZ ? 0 : S::PH - 1;
-      } else {
-        k -= 2 * XY + 2 * XZ;
-        const int q = k % YZ;
-        y = q % S::TH + 1;
-        z = q / S::TH + 1;
-        x = k < YZ ? 0 : S::PW - 1;
-      }
-      return x + S::PW * (y + S::PH * z);
-    }
-  }
-};
 
 template <class S, bool FIRST>
 __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
