@@ -16,7 +16,6 @@ struct Shape {
   static constexpr int PADN = PW * PH * PD;
   static constexpr int NT = TW * TH * TD;
   static constexpr int KMAX = D3 ? 6 : 8;
-  static constexpr int NCELL = (PADN + NT - 1) / NT;  // padded cells per lane
   static_assert(NT == 256, "one node per lane of a 256-lane workgroup");
 };
 

@@ -1458,11 +1458,16 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     // every load of the visit is issued before any is used (dead lanes and
     // halo cells read a valid dummy address)
     const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
-    HaloCell hc[S::NCELL];
-    int hh[S::NCELL], he[S::NCELL];
+    // halo: the ring cells, split evenly over the four waves (as in k_push)
+    HaloCell hc[kRingPerLane<S>];
+    int hp[kRingPerLane<S>];
+    int hh[kRingPerLane<S>], he[kRingPerLane<S>];
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
-      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
+      const int j = c * 64 + (tid & 63);
+      const int k = (tid >> 6) * kRingQ<S> + j;
+      hp[c] = j < kRingQ<S> && k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN;
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, hp[c], K);
       // device-coherent read: a neighbour tile settled earlier in this same
       // pass (possibly on another XCD) is seen at once, so one pass carries
       // the distance front across many tiles in list order.  Any tagged
@@ -1483,7 +1488,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     }
     s_h[L.me] = h0;
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
       if (!hc[c].halo) continue;
       int v = HINF;
       if (hc[c].live) {
@@ -1491,7 +1496,7 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
         // the neighbour still holds its start value
         v = ((unsigned)hh[c] >> 30) == (unsigned)tag ? hh[c] & HMASK : (he[c] < 0 ? 0 : HINF);
       }
-      s_h[tid + c * S::NT] = v;
+      s_h[hp[c]] = v;
     }
     __syncthreads();
     int any;
@@ -1726,11 +1731,16 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
     // every load of the visit is issued before any is used (dead lanes and
     // halo cells read a valid dummy address)
     const long long gv = valid ? node_index(I, gx, gy, gz) : 0;
-    HaloCell hc[S::NCELL];
-    unsigned char hl[S::NCELL], ho[S::NCELL];
+    // halo: the ring cells, split evenly over the four waves (as in k_push)
+    HaloCell hc[kRingPerLane<S>];
+    int hp[kRingPerLane<S>];
+    unsigned char hl[kRingPerLane<S>], ho[kRingPerLane<S>];
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
-      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, tid + c * S::NT, K);
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
+      const int j = c * 64 + (tid & 63);
+      const int k = (tid >> 6) * kRingQ<S> + j;
+      hp[c] = j < kRingQ<S> && k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN;
+      hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, hp[c], K);
       hl[c] = P.lab[hc[c].g];
       if (TAG) ho[c] = object[hc[c].g];
     }
@@ -1752,11 +1762,11 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
     if (!TAG && l0 && deficit) *cert = 1;
     s_l[L.me] = l0;
 #pragma unroll
-    for (int c = 0; c < S::NCELL; ++c) {
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
       if (!hc[c].halo) continue;
       unsigned char lv = 1;
       if (hc[c].live) lv = TAG ? (hl[c] ? 1 : ho[c] ? 0 : 4) : hl[c];
-      s_l[tid + c * S::NT] = lv;
+      s_l[hp[c]] = lv;
     }
     __syncthreads();
     int any;
@@ -1780,14 +1790,15 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
       P.lab[gv] = 1;
       if (!TAG && deficit) *cert = 1;
     }
-    for (int p = tid; p < S::PADN; p += S::NT) {
-      if (s_l[p] != 2) continue;
+    // halo cells labelled by this visit (only live ring cells can be)
+#pragma unroll
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
+      if (!hc[c].live || s_l[hp[c]] != 2) continue;
       int x, y, z, nout;
       bool inter;
-      pad_coords<S>(p, x, y, z, inter, nout);
-      if (inter) continue;
+      pad_coords<S>(hp[c], x, y, z, inter, nout);
       const int X = r.ox + x, Y = r.oy + y, Z = r.oz + z;
-      P.lab[node_index(I, X, Y, Z)] = 1;
+      P.lab[hc[c].g] = 1;
       dirty[tile_of<S>(I, X, Y, Z)] = 1;
     }
     __syncthreads();
