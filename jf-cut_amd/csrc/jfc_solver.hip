@@ -1300,36 +1300,54 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     int it = 0, any;
     do {
       // relabel (monotone; neighbours may be read before or after their own
-      // relabel in this iteration, which keeps the labelling valid)
+      // relabel in this iteration, which keeps the labelling valid).  Every
+      // LDS word a phase needs is read before any is used, so a phase waits
+      // on LDS once instead of once per direction.
       if (valid) {
         const int e = s_e[L.me];
         const int h = s_h[L.me];
         if (e > 0 && h < HINF) {
+          int cv[S::KMAX], hv[S::KMAX];
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d) {
+            cv[d] = s_c[d][L.me];
+            hv[d] = s_h[L.me + lds_off<S>(d)];
+          }
           int m = HINF;
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d)
-            if (d < K && s_c[d][L.me] > 0) m = min(m, s_h[L.me + lds_off<S>(d)] + 1);
+            if (d < K && cv[d] > 0) m = min(m, hv[d] + 1);
           if (m >= hcap) m = HINF;
           if (m > h) s_h[L.me] = m;
         }
       }
       __syncthreads();
-      // push
+      // push.  The reverse residual s_c[d^1][nb] is written only by this
+      // lane during the phase (its owner nb cannot push back along an arc
+      // this lane pushes on: heights are frozen), so it is read up front too.
       int pushed = 0;
       if (valid) {
         const int e = s_e[L.me];
         const int h = s_h[L.me];
         if (e > 0 && h < HINF) {
+          int cv[S::KMAX], hv[S::KMAX], rv[S::KMAX];
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d) {
+            const int nb = L.me + lds_off<S>(d);
+            cv[d] = s_c[d][L.me];
+            hv[d] = s_h[nb];
+            rv[d] = s_c[d ^ 1][nb];
+          }
           int rem = e;
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d) {
             if (d < K && rem > 0) {
               const int nb = L.me + lds_off<S>(d);
-              const int c = s_c[d][L.me];
-              if (c > 0 && h == s_h[nb] + 1) {
+              const int c = cv[d];
+              if (c > 0 && h == hv[d] + 1) {
                 const int f = min(rem, c);
                 s_c[d][L.me] = c - f;
-                s_c[d ^ 1][nb] += f;
+                s_c[d ^ 1][nb] = rv[d] + f;
                 atomicAdd(&s_e[nb], f);
                 rem -= f;
               }
@@ -1503,11 +1521,15 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     do {
       int ch = 0;
       if (mask) {
+        // all neighbour words read before any is used (one LDS wait)
         const int h = s_h[L.me];
+        int hv[S::KMAX];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) hv[d] = s_h[L.me + lds_off<S>(d)];
         int m = h;
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d)
-          if (mask & (1u << d)) m = min(m, s_h[L.me + lds_off<S>(d)] + 1);
+          if (mask & (1u << d)) m = min(m, hv[d] + 1);
         if (m < h) {
           s_h[L.me] = m;
           ch = 1;
@@ -1627,10 +1649,13 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
         if (!mk[j]) continue;
         const int me = Lane<S>(lane + 64 * j).me;
         const int h = s_h[me];
+        int hv[S::KMAX];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) hv[d] = s_h[me + lds_off<S>(d)];
         int m = h;
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d)
-          if (mk[j] & (1u << d)) m = min(m, s_h[me + lds_off<S>(d)] + 1);
+          if (mk[j] & (1u << d)) m = min(m, hv[d] + 1);
         if (m < h) {
           s_h[me] = m;
           ch = true;
@@ -1774,14 +1799,16 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
       int ch = 0;
       const unsigned me = s_l[L.me];
       if (mask && (me == 1u || me == 2u)) {
+        // neighbour labels read up front (one LDS wait); another lane may
+        // label the same neighbour meanwhile, and both then write the same 2
+        unsigned char lv[S::KMAX];
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) lv[d] = s_l[L.me + lds_off<S>(d)];
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d)
-          if (mask & (1u << d)) {
-            const int nb = L.me + lds_off<S>(d);
-            if (s_l[nb] == 0) {
-              s_l[nb] = 2;
-              ch = 1;
-            }
+          if ((mask & (1u << d)) && lv[d] == 0) {
+            s_l[L.me + lds_off<S>(d)] = 2;
+            ch = 1;
           }
       }
       any = block_any(ch, s_w, par);
