@@ -140,6 +140,7 @@ struct jfc_graph {
   unsigned char *d_defi = nullptr;  // tiles that may hold a deficit node
   int gr_tag = 1;                   // tag of the last global relabel's heights
   unsigned char *d_mat = nullptr;  // per tile: residual planes written this solve
+  unsigned char *d_hmat = nullptr; // per tile: heights stored this solve
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
   long long *d_red = nullptr, *d_total = nullptr;
@@ -213,6 +214,7 @@ struct jfc_graph {
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_defi); dfree(d_jf0); dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
+    dfree(d_hmat);
     dfree(d_lines);
   }
   void release() {
@@ -338,6 +340,7 @@ struct jfc_graph {
         (rc = dalloc(&d_hist, nbins, "tile histogram")) ||
         (rc = dalloc(&d_order, ntiles, "tile order")) ||
         (rc = dalloc(&d_mat, ntiles, "tile states")) ||
+        (rc = dalloc(&d_hmat, ntiles, "tile states")) ||
         (rc = dalloc(&d_lines, 3 * n1, "tile lines")))
       return rc;
     JFC_HIP(hipMemcpy(d_lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -848,6 +851,7 @@ struct Solver {
       launch_gr_init(g->shape, g->tiles(), g->planes(), g->gr_tag, g->d_defi, st);
     int rc = search(kRelabel, &s.relabel_passes);
     if (rc) return rc;
+    launch_fill_u8(g->d_hmat, g->ntiles, 1, st);  // every height is stored now
     gr_ev.push_back({a, tm.mark()});
     s.global_relabels++;
     return JFC_OK;
@@ -882,7 +886,7 @@ struct Solver {
                      g->d_sweep_state, g->d_total, s_);
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
-                  g->d_defi, o.max_local_iters, g->d_total + kIterSlots, s_);
+                  g->d_defi, g->d_hmat, o.max_local_iters, g->d_total + kIterSlots, s_);
       if (timed && o.profile) push_ev.push_back({a, tm.mark()});
       advance();
     }
@@ -961,10 +965,14 @@ struct Solver {
     JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
     launch_fill_u8(g->d_mat, g->ntiles, 0, st);
+    // no heights stored yet: the push kernel reads them as 0 (the Fast-Cut
+    // kernels read every height, so the fold writes the zeros for them)
+    launch_fill_u8(g->d_hmat, g->ntiles, 0, st);
     // Fast-Cut reads and writes the residual planes directly
     if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     if (int rc0 = g->ensure_pin()) return rc0;
-    launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi, st);
+    launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi,
+                o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, st);
     // the fold's sums are read with the results (no host wait here): an
     // instance without terminal capacity has no excess, so the solve passes
     // over it, and the error is reported at the end
@@ -1254,7 +1262,7 @@ extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
   const int nI = (int)g->inst.size();
   launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, 0);
   launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr,
-              g->d_defi, 0);
+              g->d_defi, false, 0);
   unsigned char *src = which == 0 ? g->d_defi : g->d_flag;
   if (which == 1) {
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, 0);

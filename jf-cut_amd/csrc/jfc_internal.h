@@ -105,8 +105,11 @@ void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t 
 
 // active (or null): tiles holding an excess node; defi: tiles holding a
 // deficit node (the relabel's jump-flooding seeds).  Both zeroed here.
+// zero_h: also write h = 0 over all nodes (the Fast-Cut kernels read every
+// height; the push kernel instead treats the heights of tiles it has not
+// visited in the solve as 0, see launch_push).
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, unsigned char *defi, hipStream_t st);
+                 unsigned char *active, unsigned char *defi, bool zero_h, hipStream_t st);
 void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
@@ -148,12 +151,14 @@ void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mas
 // mat: per tile, 1 once every residual plane of the tile was written in
 // this solve (cleared when a solve starts).  defi: a visit rewrites its
 // tile's deficit flag (pushes never create deficits, so unvisited tiles'
-// flags stay a superset).  iter_total: 64 int64 counters of local
-// iterations (or null).
+// flags stay a superset).  hmat: per tile, 1 once its heights are stored in
+// this solve (by a push visit, which then writes all of them, or by a global
+// relabel); heights of a tile with hmat 0 read as 0, the labelling a solve
+// starts from.  iter_total: 64 int64 counters of local iterations (or null).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, int max_iters, long long *iter_total,
-                 hipStream_t st);
+                 unsigned char *defi, unsigned char *hmat, int max_iters,
+                 long long *iter_total, hipStream_t st);
 // Fast-Cut comparator (jfc_fastcut.hip)
 void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);
 void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,

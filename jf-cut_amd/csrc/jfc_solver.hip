@@ -364,15 +364,17 @@ static int row_grid(int nrows) { return nrows < 2048 ? (nrows > 0 ? nrows : 1) :
 
 // Fold terminal arcs: e = cs - ct; const += min(cs, ct);
 // deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
-// With `active` set it also starts the labelling at zero heights (the
-// reference clears h, QGCWidget.cpp:2400) and flags the tiles holding excess
-// (`active` zeroed by the launcher).  It always flags the tiles holding a
+// With `active` set it flags the tiles holding excess (`active` zeroed by the
+// launcher); with `zero_h` it starts the labelling at zero heights over all
+// nodes (the reference clears h, QGCWidget.cpp:2400; the push kernel reads
+// the heights of tiles it has not visited in the solve as 0 instead).  It always flags the tiles holding a
 // deficit node in `defi`, the seeds of a global relabel's tile ordering (the
 // seed flags of graphcut_init_global_relabel, graphcut_global_relabel.cl:
 // 19-48, kept current by the push visits instead of a pass over e).
 template <class S>
 __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
-                                              unsigned char *active, unsigned char *defi) {
+                                              unsigned char *active, unsigned char *defi,
+                                              int zero_h) {
   __shared__ long long buf[4];
   long long *dst[3] = {red + RED_CONST, red + RED_DEF0, red + RED_SUP0};
   const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
         e.x = a[u].x - b[u].x; e.y = a[u].y - b[u].y;
         e.z = a[u].z - b[u].z; e.w = a[u].w - b[u].w;
         store4(P.e, v, g[u].n, e);
-        if (active) store4(P.h, v, g[u].n, make_int4(0, 0, 0, 0));
+        if (zero_h) store4(P.h, v, g[u].n, make_int4(0, 0, 0, 0));
         unsigned bits = 0, dbits = 0;
         for (int i = 0; i < g[u].n; ++i) {
           const int s = comp(a[u], i), t = comp(b[u], i);
@@ -419,11 +421,11 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
 }
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, unsigned char *defi, hipStream_t st) {
+                 unsigned char *active, unsigned char *defi, bool zero_h, hipStream_t st) {
   if (active) launch_fill_u8(active, T.ntiles, 0, st);
   launch_fill_u8(defi, T.ntiles, 0, st);
-  JFC_DISPATCH(shape,
-               (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active, defi)));
+  JFC_DISPATCH(shape, (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active, defi,
+                                                                     zero_h ? 1 : 0)));
 }
 
 // graphcut_init_cut (graphcut_compute_cut.cl:19-49): label = 1 where
@@ -1206,8 +1208,8 @@ template <class S>
 // 8, where it spills).
 __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
-                                              unsigned char *defi, int max_iters,
-                                              long long *iter_total) {
+                                              unsigned char *defi, unsigned char *hmat,
+                                              int max_iters, long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -1259,10 +1261,23 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       const int k = (tid >> 6) * kRingQ<S> + j;
       hp[c] = j < kRingQ<S> && k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN;
       hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, hp[c], K);
-      hh[c] = P.h[hc[c].g] & HMASK;
+      // a neighbour tile without stored heights in this solve reads as 0
+      int ht = t;
+      if (hc[c].live) {
+        int x, y, z, nout;
+        bool inter;
+        pad_coords<S>(hp[c], x, y, z, inter, nout);
+        ht = tile_of<S>(I, r.ox + x, r.oy + y, r.oz + z);
+      }
+      const int hw = P.h[hc[c].g];
+      hh[c] = hmat[ht] ? hw & HMASK : 0;
       hm[c] = P.rmask[hc[c].g];
     }
-    int e0 = P.e[gv], h0 = P.h[gv] & HMASK;
+    // heights not yet stored in this solve read as 0 (the start labelling);
+    // the visit then stores all of them
+    const bool h_stored = hmat[t] != 0;
+    const int hw0 = P.h[gv];
+    int e0 = P.e[gv], h0 = h_stored ? hw0 & HMASK : 0;
     unsigned rm = 0;
     int c0r[S::KMAX];
     if (!fresh_tile) {
@@ -1368,7 +1383,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     if (valid) {
       const int e1 = s_e[L.me], h1 = s_h[L.me];
       if (e1 != e0) P.e[gv] = e1;
-      if (h1 != h0) P.h[gv] = h1;
+      if (h1 != h0 || !h_stored) P.h[gv] = h1;
       // fresh planes are written whether or not they changed; the mask word
       // of a written tile holds residual bits only (exact, no fresh bits)
       unsigned m0 = 0, m1 = 0;
@@ -1433,6 +1448,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       defi[t] = (unsigned char)(ad >> 1);
     }
     if (fresh_tile && tid == 0) mat[t] = 1;
+    if (!h_stored && tid == 0) hmat[t] = 1;
   }
   // one atomic per workgroup, spread over 64 counters
   if (tid == 0 && iter_total && my_iters)
@@ -1441,10 +1457,10 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, int max_iters, long long *iter_total,
-                 hipStream_t st) {
+                 unsigned char *defi, unsigned char *hmat, int max_iters,
+                 long long *iter_total, hipStream_t st) {
   JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
-                          T, P, q, flag, mat, defi, max_iters, iter_total)));
+                          T, P, q, flag, mat, defi, hmat, max_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
