@@ -1616,14 +1616,16 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
     // the lane's nodes
     int hh[NR], he[NR];
     bool hlive[NR];
+    long long hg[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const int k = lane + 64 * i;
       const HaloCell hc =
           halo_cell<S>(I, r.ox, r.oy, r.oz, k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN, K);
       hlive[i] = hc.live;
+      hg[i] = hc.g;
       hh[i] = __hip_atomic_load(&P.h[hc.g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      he[i] = first ? P.e[hc.g] : 0;
+      he[i] = 0;
     }
     int h0[NJ];
     unsigned mk[NJ];
@@ -1644,6 +1646,14 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
         mk[j] = 0;
         pos[j] = false;
       }
+    }
+    // first pass: the excess of a halo node is needed only where its height
+    // word is stale, so it is loaded after the word and only there (most of
+    // a 2D tile's halo loads are side-column cells, one 128-byte line each)
+    if (first) {
+#pragma unroll
+      for (int i = 0; i < NR; ++i)
+        if (hlive[i] && ((unsigned)hh[i] >> 30) != (unsigned)tag) he[i] = P.e[hg[i]];
     }
 #pragma unroll
     for (int j = 0; j < NJ; ++j) s_h[Lane<S>(lane + 64 * j).me] = h0[j];
