@@ -22,6 +22,7 @@
 #include "jfc_device.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 
 namespace jfc {
@@ -1107,6 +1108,24 @@ struct QueueCursor {
 // still runs its static first run after the others are done, stretching the
 // launch -- so the grid is capped at what the device holds at once for this
 // kernel (occupancy per CU, from its register and LDS use, times CUs).
+// Compute units of the current device, queried once per device (every
+// persistent launch sizes its grid with it).
+static int device_cus() {
+  static std::atomic<int> cache[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= 0 && dev < 16) {
+    const int c = cache[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  if (dev >= 0 && dev < 16) cache[dev].store(cus, std::memory_order_relaxed);
+  return cus;
+}
+
 template <auto Kernel, int Block = 256>
 static int resident_grid_b(int want) {
   static const int per_cu = [] {
@@ -1115,12 +1134,7 @@ static int resident_grid_b(int want) {
       n = 1;
     return n;
   }();
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus < 1)
-    cus = 1;
-  return std::max(1, std::min(want, per_cu * cus));
+  return std::max(1, std::min(want, per_cu * device_cus()));
 }
 
 template <auto Kernel>
