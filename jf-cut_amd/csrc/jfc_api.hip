@@ -86,7 +86,8 @@ int shape_from_option(int opt, int dim) {
 }
 
 // Process-wide autotuning table: size class -> JFC_TILE_* shape.
-// Autotuned choices per size class: tile shape and local-iteration cap.
+// Autotuned choices per size class: tile shape, local-iteration cap and the
+// relabel before the first sweep.
 struct Tuned {
   int shape = 0;
   int mli = 0;
@@ -179,8 +180,10 @@ struct jfc_graph {
   int ensure_pin() {
     if (pin) return JFC_OK;
     for (hipEvent_t &ev : cnt_ev)
-      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+      if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
         return fail(JFC_ERR_DEVICE, "event creation failed");
+      }
     const size_t nres = inst.size() * (RED_SLOTS + 1) + kTotals;
     if (hipHostMalloc((void **)&pin, sizeof(HostPin), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&pin_res, nres * sizeof(long long), hipHostMallocDefault) !=
