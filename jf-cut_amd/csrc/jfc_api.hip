@@ -1289,12 +1289,12 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   const int nc = g->dim == 3 ? 2 : 3;
   // the local-iteration cap is tuned too unless the caller fixed it: fronts
   // that cross tiles in few visits favour more iterations per visit
-  // (config 4: 26 -> 20 ms at 4), others favour 2 (config 3: 17 vs 31 ms)
+  // (config 4: 21.6 -> 14.3 ms at 4), others favour 2 (config 3)
   const int mli_fixed = o.max_local_iters > 0 ? o.max_local_iters : 0;
   const int mcand[3] = {2, 3, 4};
   const int nm = mli_fixed ? 1 : 3;
   // so is the relabel before the first sweep: it pays where zero heights
-  // let flow slosh for sweeps (config 3: 13.4 -> 10.2 ms) and costs where
+  // let flow slosh for sweeps (config 3: 12.2 -> 9.4 ms) and costs where
   // the push sweeps settle most of the flow first (config 2: 4.8 -> 8.5 ms)
   const int ir_fixed = o.initial_relabel >= 0 ? o.initial_relabel : -1;
   const int ni = ir_fixed >= 0 ? 1 : 2;
