@@ -953,11 +953,20 @@ struct Solver {
       g->sweep_cap = o.max_sweeps + 1;
     }
     hipEvent_t t0 = tm.mark();
-    launch_fill_i32(g->d_ctr, 2 * QSLOT, 0, st);
-    launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, st);
-    launch_fill_i32((int *)g->d_total, 2 * kTotals, 0, st);
-    launch_fill_i32(g->d_sweep, o.max_sweeps + 1, 0, st);
-    launch_fill_i32(g->d_sweep_state, 2, 0, st);
+    // counters and per-tile states in one launch (the sweep history needs no
+    // clearing: each sweep files its entry before any read of it); no
+    // heights stored yet, so the push kernel reads them as 0 (the Fast-Cut
+    // kernels read every height, and the fold writes the zeros for them)
+    {
+      FillSet f;
+      f.add(g->d_ctr, 2 * QSLOT, 0, 4);
+      f.add(g->d_red, 2LL * nI * RED_SLOTS, 0, 4);
+      f.add(g->d_total, 2 * kTotals, 0, 4);
+      f.add(g->d_sweep_state, 2, 0, 4);
+      f.add(g->d_mat, g->ntiles, 0, 1);
+      f.add(g->d_hmat, g->ntiles, 0, 1);
+      launch_fill_multi(f, st);
+    }
     g->slot = 0;
     // the residual-arc masks restart from the inputs' with every plane
     // marked fresh: k_push reads fresh planes from c0 and writes them to cap
@@ -967,10 +976,6 @@ struct Solver {
     }
     JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
-    launch_fill_u8(g->d_mat, g->ntiles, 0, st);
-    // no heights stored yet: the push kernel reads them as 0 (the Fast-Cut
-    // kernels read every height, so the fold writes the zeros for them)
-    launch_fill_u8(g->d_hmat, g->ntiles, 0, st);
     // Fast-Cut reads and writes the residual planes directly
     if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     if (int rc0 = g->ensure_pin()) return rc0;
@@ -983,7 +988,7 @@ struct Solver {
     int rc;
     if (o.initial_relabel) {
       if ((rc = global_relabel())) return rc;
-    }  // else k_fold already set zero heights (a valid labelling) and the flags
+    }  // else the zero heights (a valid labelling) and the fold's flags
     long long total_at_gr = 0;
     int since_check = 0;
     int cert_rounds = 0;

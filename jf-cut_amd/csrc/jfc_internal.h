@@ -102,6 +102,21 @@ void set_last_error(const std::string &msg);
 // ---- launchers (all asynchronous on `st`) --------------------------------
 void launch_fill_i32(int *p, long long n, int v, hipStream_t st);
 void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t st);
+// Up to 8 fills in one launch (bytes = 1 or 4 per element).
+struct FillSeg {
+  void *p;
+  long long n;
+  int v;
+  int bytes;
+};
+struct FillSet {
+  FillSeg seg[8];
+  int n = 0;
+  void add(void *p, long long count, int v, int bytes) {
+    if (count > 0 && n < 8) seg[n++] = FillSeg{p, count, v, bytes};
+  }
+};
+void launch_fill_multi(const FillSet &f, hipStream_t st);
 
 // active (or null): tiles holding an excess node; defi: tiles holding a
 // deficit node (the relabel's jump-flooding seeds).  Both zeroed here.

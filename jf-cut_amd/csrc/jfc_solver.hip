@@ -63,6 +63,24 @@ void launch_fill_u8(unsigned char *p, long long n, unsigned char v, hipStream_t 
   if (n > 0) k_fill_u8<<<grid_for(n), 256, 0, st>>>(p, n, v);
 }
 
+// Several fills in one launch: blockIdx.y picks the segment.
+__global__ void k_fill_multi(FillSet f) {
+  if ((int)blockIdx.y >= f.n) return;
+  const FillSeg s = f.seg[blockIdx.y];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < s.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (s.bytes == 1) static_cast<unsigned char *>(s.p)[i] = (unsigned char)s.v;
+    else static_cast<int *>(s.p)[i] = s.v;
+  }
+}
+
+void launch_fill_multi(const FillSet &f, hipStream_t st) {
+  if (f.n <= 0) return;
+  long long most = 1;
+  for (int k = 0; k < f.n; ++k) most = std::max(most, f.seg[k].n);
+  k_fill_multi<<<dim3(grid_for(most, 256, 1024), f.n), 256, 0, st>>>(f);
+}
+
 __global__ void k_validate(const int *p, long long n, int lo, int hi, int *bad) {
   int b = 0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
