@@ -5,8 +5,13 @@ Default workload (the large 2D cut): 15000x10000 RGB uint8 image from the
 synthetic generator (40 colour blobs, noise sigma 25, scribble seeds of
 capacity 2^30), integer k-means data term (K=5 per side), 8-connected grid with
 pairwise round(50 exp(-|dI|^2 / 2 s2)) (diagonals / sqrt 2), int32 capacities.
-A step is one full solve from the capacities resident in HBM to the exact int64
-flow and the canonical labels resident in HBM.
+
+A step is one full solve of freshly written inputs: the capacities are
+resident in HBM (written in place, jfc_inputs_written), and the timed solve
+rebuilds everything derived from them (residual-arc masks, folded excess) and
+runs to the exact int64 flow and the canonical labels in HBM.  A warm re-solve
+of unchanged inputs and a step that also copies the inputs in from caller
+buffers are reported beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -14,6 +19,7 @@ import argparse
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -28,11 +34,16 @@ METRIC = "Mvoxels/s to exact max-flow on 1 GPU; HBM GB/s; batched fwd+bwd vs pla
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "c2": dict(W=15000, H=10000, conn=8, nblobs=40, noise=25.0, lam=50.0, k=5,
+    "c2": dict(W=15000, H=10000, conn=8, nblobs=40, noise=25.0, lam=50.0, k=5, unary=100,
                name="15000x10000 RGB 8-connected grid cut, k-means K=5, lambda 50, int32"),
-    "c1": dict(W=512, H=512, conn=4, nblobs=8, noise=25.0, lam=50.0, k=5,
+    "c1": dict(W=512, H=512, conn=4, nblobs=8, noise=25.0, lam=50.0, k=5, unary=100,
                name="512x512 RGB 4-connected grid cut, k-means K=5, lambda 50, int32"),
 }
+# the published max-flows of the reference's own graphs
+# (latex/sources/section-5-results.tex:31,36,41) and the reference's JF-Cut
+# total for flower on a GTX TITAN (benchmark/performance.xlsx, sheet CPU, O3)
+FIXTURE_FLOWS = {"flower": 1208089, "person": 880461, "sponge": 343591}
+FLOWER_TITAN_MS = 5.70
 
 
 def hip_sync():
@@ -51,7 +62,8 @@ def build_graph(wl, seed=0, W=None, H=None):
     H = H or wl["H"]
     rgb, sd = jfcut.synth_image(W, H, nblobs=wl["nblobs"], noise=wl["noise"], seed=seed)
     g = jfcut.GridGraph.grid(W, H, 1, wl["conn"])
-    g.model_image(rgb, sd, k_fg=wl["k"], k_bg=wl["k"], lambda_xy=wl["lam"])
+    g.model_image(rgb, sd, k_fg=wl["k"], k_bg=wl["k"], lambda_xy=wl["lam"],
+                  unary_scale=wl["unary"])
     return g
 
 
@@ -74,6 +86,44 @@ class DeviceBuffer:
             self.p = ctypes.c_void_p()
 
 
+def timed_steps(g, steps, fresh, setter=None, **opt):
+    """Time `steps` solves (wall clock, device synchronised on both sides).
+    fresh: mark the inputs written before each solve; setter: callable that
+    copies the inputs in before each solve (inside the timed region)."""
+    stats = []
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if setter is not None:
+            setter()
+        elif fresh:
+            g.inputs_written()
+        st = g.solve(**opt)
+        stats.append(st)
+    hip_sync()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    return ms, stats
+
+
+def device_setter(g, n, K):
+    """Copies of the graph's inputs in device buffers and a setter that
+    copies them back in through the C-ABI (device pointers), validation
+    included."""
+    L = jfcut.lib()
+    bufs = [DeviceBuffer(n * 4), DeviceBuffer(n * 4)] + [DeviceBuffer(n * 4) for _ in range(K)]
+    vp = lambda b: ctypes.c_void_p(b.data_ptr())
+    jfcut._check(L.jfc_get_terminal_caps(g._h, 0, vp(bufs[0]), vp(bufs[1]), 1))
+    for d in range(K):
+        jfcut._check(L.jfc_get_neighbor_caps(g._h, 0, d, vp(bufs[2 + d]), 1))
+
+    def setter():
+        jfcut._check(L.jfc_set_terminal_caps(g._h, 0, vp(bufs[0]), vp(bufs[1]), 1))
+        for d in range(K):
+            jfcut._check(L.jfc_set_neighbor_caps(g._h, 0, d, vp(bufs[2 + d]), 1))
+
+    return setter, bufs
+
+
 def batched_forward_backward(n=256, reps=3):
     """Config 5: the 256 uneven cuts (4C/8C, lambda 20-80) solved together and
     differentiated in one backward launch, against the same cuts solved and
@@ -92,6 +142,8 @@ def batched_forward_backward(n=256, reps=3):
     gb = [DeviceBuffer(ntot * 4), DeviceBuffer(ntot * 4), DeviceBuffer(ntot * kmax * 4)]
     maxn = max(s["W"] * s["H"] for s in specs)
     gs = [DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 4), DeviceBuffer(maxn * 8 * 4)]
+    L = jfcut.lib()
+    vp = lambda b: ctypes.c_void_p(b.data_ptr())
     # tile shape per size class, chosen by the autotuner (outside the timing)
     tile = G.autotune(reps=2)
     G.solve()
@@ -101,7 +153,7 @@ def batched_forward_backward(n=256, reps=3):
     t0 = time.perf_counter()
     for _ in range(reps):
         st = G.solve()
-        G.backward_all(out=gb)
+        jfcut._check(L.jfc_backward_all(G._h, vp(gb[0]), vp(gb[1]), vp(gb[2]), 1))
     hip_sync()
     t_batch = (time.perf_counter() - t0) / reps
     flows_b = G.flows()
@@ -109,7 +161,7 @@ def batched_forward_backward(n=256, reps=3):
     for _ in range(reps):
         for g in singles:
             g.solve()
-            g.backward(out=gs)
+            jfcut._check(L.jfc_backward(g._h, 0, vp(gs[0]), vp(gs[1]), vp(gs[2]), 1))
     hip_sync()
     t_seq = (time.perf_counter() - t0) / reps
     flows_s = np.array([g.flow() for g in singles])
@@ -128,39 +180,166 @@ def batched_forward_backward(n=256, reps=3):
             "batched_sweeps": st.sweeps, "tile_shape": tile, "flows_equal": True}
 
 
-def cpu_baseline(wl, sample_w, sample_h):
-    """The reference's CPU max-flow (BK303 built from the reference sources)
-    on a bounded sample of the workload, plus the GPU solve of the same sample
-    for parity in the same run."""
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = 0
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemAvailable"):
+                avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model or platform.processor(),
+            "mem_available_gb": round(avail / 2**30, 1)}
+
+
+def cpu_baseline(wl, gflow, glab, cs, ct, caps, want_full):
+    """The reference's CPU max-flow (BK303 built from the reference sources,
+    one thread) on the workload itself when host memory allows (BK303 needs
+    about 300 bytes per node on an 8-connected grid), else on a bounded
+    sample of it; flow and labels compared with the GPU's in the same run."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     if not ol.have_bk():
         return None
-    g = build_graph(wl, seed=1, W=sample_w, H=sample_h)
-    cs, ct = g.terminal_caps()
-    caps = g.neighbor_caps()
-    st = g.solve()
-    gflow, glab = g.flow(), g.labels()
-    g.close()
-    t0 = time.perf_counter()
-    flow, lab, (ti, tm, to) = ol.bk303(sample_w, sample_h, 1, wl["conn"], cs, ct, caps,
-                                       timings=True)
-    wall = time.perf_counter() - t0
-    n = sample_w * sample_h
-    return {
+    host = host_info()
+    W, H = wl["W"], wl["H"]
+    need_gb = W * H * 320 / 2**30 + 8
+    full = want_full and host["mem_available_gb"] >= need_gb
+    reason = None
+    if full:
+        n = W * H
+        sample = f"the full {W}x{H} workload (the same capacities as the GPU solve)"
+        t0 = time.perf_counter()
+        flow, lab, (ti, tm, to) = ol.bk303(W, H, 1, wl["conn"], cs, ct, caps, timings=True)
+        wall = time.perf_counter() - t0
+        gpu_sample_ms = None
+        parity = (bool(flow == gflow), bool(np.array_equal(lab, glab)))
+    else:
+        reason = (f"host memory: {host['mem_available_gb']} GB available, BK303 needs about "
+                  f"{need_gb:.0f} GB for the full graph") if want_full else "sample requested"
+        sw, sh = W // 2, H // 2
+        n = sw * sh
+        g = build_graph(wl, seed=1, W=sw, H=sh)
+        scs, sct = g.terminal_caps()
+        scaps = g.neighbor_caps()
+        st = g.solve()
+        sflow, slab = g.flow(), g.labels()
+        g.close()
+        sample = f"a {sw}x{sh} instance of the same generator and data term (seed 1)"
+        t0 = time.perf_counter()
+        flow, lab, (ti, tm, to) = ol.bk303(sw, sh, 1, wl["conn"], scs, sct, scaps, timings=True)
+        wall = time.perf_counter() - t0
+        gpu_sample_ms = round(st.ms_total, 3)
+        parity = (bool(flow == sflow), bool(np.array_equal(lab, slab)))
+    out = {
         "value": round(n / (ti + tm + to) / 1e6, 3),
         "unit": "Mvoxels/s",
         "cores": 1,
         "kind": "reference",
-        "sample": f"BK303 (benchmark/GridCut/code/BK303, graph init + maxflow + labels) on a "
-                  f"{sample_w}x{sample_h} instance of the same generator and data term "
-                  f"({wl['conn']}-connected, seed 1)",
+        "sample": "BK303 (benchmark/GridCut/code/BK303, single thread: graph init + maxflow + "
+                  "labels, the phases of bench-BK303.cpp:34-57) on " + sample,
         "seconds": {"init": round(ti, 3), "maxflow": round(tm, 3), "output": round(to, 3),
                     "wall": round(wall, 3)},
-        "gpu_same_sample_ms": round(st.ms_total, 3),
-        "parity_flow": bool(flow == gflow),
-        "parity_labels": bool(np.array_equal(lab, glab)),
+        "host": host,
+        "parity_flow": parity[0],
+        "parity_labels": parity[1],
     }
+    if reason:
+        out["not_full_reason"] = reason
+        out["gpu_same_sample_ms"] = gpu_sample_ms
+    return out
+
+
+def fixtures_leg(reps=5):
+    """The reference's own graphs (flower 600x450, person 600x450, sponge
+    640x480, 4-connected; tests/golden/ref_*.npz hold the reference's
+    .ef/.c1/.c2 arrays): GPU ms per fresh solve and BK303 ms on the same
+    graphs in the same run, flows equal to the published values."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    out = {"published_jfcut_flower_ms_gtx_titan": FLOWER_TITAN_MS}
+    for name, flow in FIXTURE_FLOWS.items():
+        z = np.load(os.path.join(ROOT, "tests", "golden", f"ref_{name}.npz"))
+        nx, ny = int(z["nx"]), int(z["ny"])
+        ef = z["ef"].astype(np.int32)
+        caps = z["caps"].astype(np.int32)
+        g = jfcut.GridGraph([(nx, ny, 1, 4)])
+        g.set_excess(ef)
+        for d in range(4):
+            g.set_neighbor_caps(d, caps[d])
+        g.solve()
+        ms, stats = timed_steps(g, reps, fresh=True)
+        assert g.flow() == flow, (name, g.flow(), flow)
+        dev = float(np.median([s.ms_total for s in stats]))
+        g.close()
+        rec = {"nodes": nx * ny, "gpu_ms": round(ms, 3), "gpu_device_ms": round(dev, 3),
+               "flow": flow, "flow_equal_published": True}
+        if ol.have_bk():
+            cs, ct = np.maximum(ef, 0), np.maximum(-ef, 0)
+            bt = []
+            for _ in range(3):
+                f, lab, t = ol.bk303(nx, ny, 1, 4, cs, ct, caps, timings=True)
+                bt.append(sum(t))
+            assert f == flow
+            rec["bk303_ms_1_core"] = round(min(bt) * 1e3, 2)
+        out[name] = rec
+    return out
+
+
+def config_leg(name, steps=3):
+    """A 3D configuration (video volume c3 or 512^3 volume c4): autotuned,
+    then fresh-input solves; Mvoxels/s and the flow (checked equal to the cut
+    energy)."""
+    t0 = time.time()
+    g, img, sd = jfcut.build_config(name, seed=0)
+    del img, sd
+    c = jfcut.CONFIGS[name]
+    n = c["W"] * c["H"] * c.get("T", c.get("D", 1))
+    tile = g.autotune(reps=1)
+    ms, stats = timed_steps(g, steps, fresh=True)
+    flows = {s.flow for s in stats}
+    assert len(flows) == 1 and all(s.converged == 1 for s in stats)
+    assert g.energy() == stats[0].flow
+    st = stats[-1]
+    out = {"nodes": n, "ms_per_solve": round(ms, 3),
+           "mvoxels_per_s": round(n / (ms * 1e-3) / 1e6, 1),
+           "flow": int(stats[0].flow), "tile_shape": tile, "local_iters": g.local_iters(),
+           "initial_relabel": g.initial_relabel(), "sweeps": st.sweeps,
+           "global_relabels": st.global_relabels, "relabel_passes": st.relabel_passes,
+           "ms_relabel": round(st.ms_relabel, 3),
+           "whole_solve_frac": round(st.solve_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+           "build_s": round(time.time() - t0, 1)}
+    g.close()
+    return out
+
+
+def seeds_only_leg(W, H, steps=2):
+    """A harder 2D cut: the config-2 image and pairwise terms with hard seeds
+    only (unary scale 0), so flow has to travel from the foreground scribbles
+    across the image to the background scribbles."""
+    wl = dict(WORKLOADS["c2"], W=W, H=H, unary=0)
+    g = build_graph(wl, seed=0)
+    g.solve()
+    ms, stats = timed_steps(g, steps, fresh=True)
+    st = stats[-1]
+    assert all(s.converged == 1 for s in stats) and len({s.flow for s in stats}) == 1
+    assert g.energy() == st.flow
+    out = {"workload": f"{W}x{H} RGB 8-connected, seeds only (unary scale 0), lambda 50",
+           "nodes": W * H, "ms_per_solve": round(ms, 3),
+           "mvoxels_per_s": round(W * H / (ms * 1e-3) / 1e6, 1), "flow": int(st.flow),
+           "sweeps": st.sweeps, "global_relabels": st.global_relabels,
+           "push_tiles": int(st.push_tiles),
+           "whole_solve_frac": round(st.solve_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    g.close()
+    return out
 
 
 def fastcut_comparator(reps=5):
@@ -187,16 +366,21 @@ def fastcut_comparator(reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", action="store_true",
+                    help="time BK303 on a quarter-size sample instead of the full workload")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-comparator", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-fixtures", action="store_true")
+    ap.add_argument("--no-seeds-only", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--cpu-sample", default="7500x5000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
@@ -216,35 +400,60 @@ def main():
 
     wl = WORKLOADS[args.workload]
     N = wl["W"] * wl["H"]
+    K = wl["conn"]
     t0 = time.time()
     g = build_graph(wl, seed=0)
     log(f"workload built in {time.time() - t0:.1f}s")
+    tile = None
+    if not args.no_autotune:
+        tile = g.autotune(reps=2)
+        log(f"autotuned: tile {tile}, local iterations {g.local_iters()}, "
+            f"initial relabel {g.initial_relabel()}")
     for i in range(args.warmup):
+        g.inputs_written()
         st = g.solve()
-        log(f"warmup {i}: {st.ms_total:.1f} ms, sweeps {st.sweeps}, flow {st.flow}")
-    flows, stats = [], []
-    hip_sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = g.solve(profile=1)
-        flows.append(st.flow)
-        stats.append(st)
-        log(f"step {i}: {st.ms_total:.1f} ms device, push {st.ms_push:.1f} ms, "
-            f"sweeps {st.sweeps}, relabels {st.global_relabels}")
-    hip_sync()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
+        log(f"warmup {i}: {st.ms_total:.2f} ms, sweeps {st.sweeps}, flow {st.flow}")
+
+    # headline: fresh inputs, push launches timed with HIP events on the solve
+    # stream (profile = 1 launches each sweep eagerly)
+    ms_per_step, stats = timed_steps(g, args.steps, fresh=True, profile=1)
+    for i, st in enumerate(stats):
+        log(f"step {i}: {st.ms_total:.2f} ms device, push {st.ms_push:.2f} ms, "
+            f"sweeps {st.sweeps}, relabels {st.global_relabels}, fresh {st.inputs_fresh}")
+    flows = [s.flow for s in stats]
     assert len(set(flows)) == 1, flows
-    assert all(s.converged == 1 for s in stats)
+    assert all(s.converged == 1 and s.inputs_fresh == 1 for s in stats)
     energy = g.energy()
     assert energy == flows[0], (energy, flows[0])  # weak duality: cut == flow
     push_bytes = sum(s.push_bytes for s in stats)
     push_ms = sum(s.ms_push for s in stats)
     push_launches = sum(s.push_launches for s in stats)
     achieved = push_bytes / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0
+    solve_bytes = sum(s.solve_bytes for s in stats) / len(stats)
+    dev_ms = sum(s.ms_total for s in stats) / len(stats)
+
+    # the same solve without events (captured sweep graphs), fresh inputs
+    plain_ms, _ = timed_steps(g, args.steps, fresh=True)
+    # warm re-solve: unchanged inputs, masks kept from the previous solve
+    warm_ms, wst = timed_steps(g, args.steps, fresh=False)
+    assert all(s.inputs_fresh == 0 for s in wst)
+    # inputs copied in from caller device buffers (validation included)
+    setter, bufs = device_setter(g, N, K)
+    set_ms, _ = timed_steps(g, max(2, args.steps // 2), fresh=False, setter=setter)
+    for b in bufs:
+        b.free()
+    log(f"fresh {ms_per_step:.3f} ms (no events {plain_ms:.3f}), warm {warm_ms:.3f} ms, "
+        f"with device setters {set_ms:.3f} ms")
+    glab = g.labels()
+    cs = ct = caps = None
+    if not args.no_cpu_baseline:
+        cs, ct = g.terminal_caps()
+        caps = g.neighbor_caps()
     g.close()
-    # HBM bytes per k_push dispatch from the calibrated rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs of this workload (profiles/k_push_c2_15000x10000_8c_traffic.md)
+
+    # HBM bytes per k_push dispatch, from the calibrated rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload committed under profiles/ (counters are
+    # not readable from inside this process)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_push_c2_traffic.json")
     if args.workload == "c2" and os.path.exists(tpath):
@@ -252,6 +461,27 @@ def main():
             kt = [v for k, v in json.load(f)["kernels"].items() if "k_push" in k]
         if kt:
             traffic = kt[0]["read_bytes_per_dispatch"] + kt[0]["write_bytes_per_dispatch"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, flows[0], glab, cs, ct, caps, want_full=not args.cpu_sample)
+        log(f"cpu baseline: {cpu}")
+        del cs, ct, caps
+
+    fixtures = None
+    if not args.no_fixtures:
+        fixtures = fixtures_leg()
+        log(f"fixtures: {fixtures}")
+
+    seeds_only = None
+    if not args.no_seeds_only and args.workload == "c2":
+        seeds_only = seeds_only_leg(wl["W"], wl["H"])
+        log(f"seeds only: {seeds_only}")
+
+    volumes = None
+    if not args.no_3d:
+        volumes = {name: config_leg(name) for name in ("c3", "c4")}
+        log(f"3D configs: {volumes}")
 
     batched = None
     if not args.no_batched:
@@ -262,11 +492,6 @@ def main():
     if not args.no_comparator:
         comparator = fastcut_comparator()
         log(f"fast-cut comparator: {comparator}")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        sw, sh = [int(v) for v in args.cpu_sample.split("x")]
-        cpu = cpu_baseline(wl, sw, sh)
 
     line = {
         "metric": METRIC,
@@ -281,17 +506,31 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (deterministic generator, seed 0)",
-        "config": {"workload": wl["name"], "nodes": N, "connectivity": wl["conn"],
-                   "flow": int(flows[0]), "device_ms_per_step": round(
-                       sum(s.ms_total for s in stats) / len(stats), 3),
-                   "sweeps": stats[-1].sweeps, "global_relabels": stats[-1].global_relabels,
+        "config": {"workload": wl["name"], "nodes": N, "connectivity": K,
+                   "step": "fresh inputs: capacities written in HBM, every derived "
+                           "structure rebuilt inside the timed solve",
+                   "flow": int(flows[0]), "device_ms_per_step": round(dev_ms, 3),
+                   "ms_no_events": round(plain_ms, 3), "ms_warm_resolve": round(warm_ms, 3),
+                   "ms_with_device_setters": round(set_ms, 3),
+                   "mvoxels_per_s_warm": round(N / (warm_ms * 1e-3) / 1e6, 1),
+                   "tile_shape": tile, "sweeps": stats[-1].sweeps,
+                   "global_relabels": stats[-1].global_relabels,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": "profiles/k_push_c2_traffic.json (rocprofv3 --pmc "
+                                       "FETCH_SIZE / WRITE_SIZE of this workload, calibrated)",
                      "kernel": "k_push (block-wise push-relabel)",
                      "bytes_per_launch": int(push_bytes / max(1, push_launches)),
-                     "avg_launch_ms": round(push_ms / max(1, push_launches), 4)},
+                     "avg_launch_ms": round(push_ms / max(1, push_launches), 4),
+                     "whole_solve": {
+                         "bytes": int(solve_bytes),
+                         "achieved": round(solve_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                         "frac": round(solve_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}},
         "cpu_baseline": cpu,
+        "fixtures": fixtures,
+        "seeds_only_2d": seeds_only,
+        "volumes_3d": volumes,
         "batched_fwd_bwd": batched,
         "fastcut_comparator": comparator,
     }

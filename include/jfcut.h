@@ -31,8 +31,13 @@
  *     jfc_solve, which may also return JFC_NOT_CONVERGED (1).  The message of
  *     the last error of the calling thread is returned by jfc_last_error().
  *   - A graph may be used by one host thread at a time; distinct graphs are
- *     independent.  Work is issued on the stream given in jfc_options (NULL =
- *     the default stream) and every call returns after that work completed.
+ *     independent.  Streams: every call of a graph issues its device work on
+ *     the graph's stream (jfc_set_stream; NULL = the legacy default stream),
+ *     ordered after the work the caller queued there before the call, and
+ *     returns after that work completed (it waits for that stream only, not
+ *     for the whole device).  jfc_solve uses jfc_options.stream instead when
+ *     that is set.  Device pointers passed in are read in that stream order:
+ *     a caller producing inputs on another stream must order it first.
  */
 #ifndef JFCUT_H
 #define JFCUT_H
@@ -126,6 +131,12 @@ typedef struct {
   double ms_relabel;        /* global relabel phases */
   double ms_cut;            /* canonical cut + certificate */
   int64_t local_iters;      /* LDS discharge iterations summed over tile visits */
+  int64_t solve_bytes;      /* algorithmic HBM bytes of every kernel of the solve:
+                               per-node passes (input masks, fold, cut start) at
+                               their bytes per node, tile kernels at their bytes
+                               per node of each visited tile */
+  int inputs_fresh;         /* 1 if the inputs changed since the last solve (the
+                               residual-arc masks were rebuilt in this solve) */
 } jfc_stats;
 
 /* ---- lifetime ------------------------------------------------------------ */
@@ -149,6 +160,13 @@ int jfc_create_batch(jfc_graph **g, int count, const int32_t *shapes);
 void jfc_destroy(jfc_graph *g);
 
 int jfc_num_instances(const jfc_graph *g);
+
+/* The stream (hipStream_t) every later call of the graph issues its work on;
+ * NULL = the legacy default stream (the initial setting).  The reference's
+ * single in-order command queue (QGCW:808) becomes the caller's stream, so a
+ * torch or HIP pipeline that produces capacities on it needs no device-wide
+ * synchronisation. */
+int jfc_set_stream(jfc_graph *g, void *stream);
 
 /* Thread-local message of the last error (QError / cl::Error text,
  * infrastructures/QError.h:37-74, QGCW:2533-2557). */
@@ -176,6 +194,23 @@ int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int32_t *cap,
  * graphcut_compute_excessflow.cl:71-74. */
 int jfc_set_seeds(jfc_graph *g, int inst, const uint8_t *fg, const uint8_t *bg,
                   int on_device);
+
+/* Zero-copy inputs: device pointers to instance `inst`'s stored input planes,
+ * for callers whose own kernels produce capacities in place (the reference's
+ * modeling kernels write devExcessFlow / devCapacity directly, QGCW:2560-2606).
+ * Node (x, y, z) of the instance is element x + row_pitch * (y + ny * z) of
+ * cap_source, cap_sink and of each neighbour plane d at
+ * neighbor + d * plane_stride; row_pitch >= nx, and the elements of a row
+ * past nx must stay 0.  After writing, call jfc_inputs_written: it marks the
+ * instance's inputs as set and drops every cache derived from them (the
+ * residual-arc masks), so the next solve starts from the new values.  The
+ * library does not range-check values written this way: sources and sinks
+ * must lie in [0, JFC_MAX_TERMINAL_CAP], neighbour capacities in
+ * [0, JFC_MAX_NEIGHBOR_CAP], and arcs leaving the grid must be 0.  Any
+ * output pointer may be NULL. */
+int jfc_input_planes(jfc_graph *g, int inst, int32_t **cap_source, int32_t **cap_sink,
+                     int32_t **neighbor, int64_t *row_pitch, int64_t *plane_stride);
+int jfc_inputs_written(jfc_graph *g, int inst);
 
 /* ---- solve (QGCW:2364-2558 slotCutClicked) --------------------------------- */
 

@@ -124,6 +124,8 @@ struct jfc_graph {
   size_t stage_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<int> caps_set;  // per instance: bit 0 terminals, bit 1 + d neighbour planes
+  std::vector<int> caps_failed;  // per instance: the same bits, for a setter that failed
+                                 // (its planes are zeroed; solves refuse the instance)
   // tile tables for the current shape
   int ntiles = 0, ncolors = 2, max_tile_dim = 1, nbins = 2;
   std::vector<int> color_off;
@@ -146,6 +148,7 @@ struct jfc_graph {
       *d_order = nullptr;
   long long *d_red = nullptr, *d_total = nullptr;
   long long *d_def = nullptr;  // per instance: remaining deficit after the cut
+  long long *d_energy = nullptr;  // per instance: cut energy (jfc_get_energy)
   int *d_row_off = nullptr;    // per instance: first row; [count] = rows
   int *d_lines = nullptr;      // per axis: prefix count of tile lines per instance
   int lines_total[3] = {0, 0, 0};
@@ -165,6 +168,10 @@ struct jfc_graph {
   int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
   hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
   hipEvent_t priv_event = nullptr;
+  // The caller's stream for every call of the graph (jfc_set_stream): the
+  // setters, the data term, the outputs and the backward issue their work
+  // on it and wait for that stream alone.  NULL: the legacy default stream.
+  hipStream_t ustream = nullptr;
   // Pinned host words the solve's decisions read (asynchronous copies, then
   // a wait on an event instead of a staged pageable copy).
   struct HostPin {
@@ -172,6 +179,8 @@ struct jfc_graph {
     int cnt[2];       // last list count of two pass groups (searches)
     int recent[3];    // active tiles of the latest sweeps
     int flag;         // certificate word of a cut search
+    int bad;          // validation word of a setter
+    long long red[RED_SLOTS];  // one instance's sums (energy)
   } *pin = nullptr;
   // Pinned copies of a solve's results: per-instance sums (RED_SLOTS each),
   // remaining deficits, then the kTotals counters.
@@ -224,7 +233,7 @@ struct jfc_graph {
     release_tiles();
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
     dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
-    dfree(d_def); dfree(d_row_off); dfree(d_sweep_state);
+    dfree(d_def); dfree(d_energy); dfree(d_row_off); dfree(d_sweep_state);
     if (stage) (void)hipFree(stage);
     stage = nullptr;
     stage_bytes = 0;
@@ -374,6 +383,7 @@ int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int 
 
 void graph_mark_inputs(jfc_graph *g, int inst) {
   g->caps_set[inst] = ~0;
+  g->caps_failed[inst] = 0;
   g->mask_dirty = true;
   g->solved = false;
 }
@@ -440,6 +450,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
   g->Ntot = off;
   g->Nuser = uoff;
   g->caps_set.assign(count, 0);
+  g->caps_failed.assign(count, 0);
   g->flows.assign(count, 0);
   int rc;
   const size_t N = (size_t)off;
@@ -454,6 +465,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
       (rc = dalloc(&g->d_total, kTotals, "counters")) ||
       (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
+      (rc = dalloc(&g->d_energy, (size_t)count, "reductions")) ||
       (rc = dalloc(&g->d_sweep_state, 2, "sweep counters")) ||
       (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
@@ -548,7 +560,9 @@ static int check_inst(jfc_graph *g, int inst) {
 // Caller arrays (rows of nx) <-> stored planes (rows of px); pad entries of
 // the stored side are left as they are (zero from creation).  Host arrays
 // cross PCIe in one contiguous copy through the graph's device staging
-// buffer; the row scatter / gather is a device-to-device 2D copy.
+// buffer; the row scatter / gather is a device-to-device 2D copy.  Every copy
+// and kernel of an entry point is issued on the graph's stream (ustream), and
+// the entry point waits for that stream alone before it returns (sync_user).
 static int stage_for(jfc_graph *g, size_t bytes, void **p) {
   if (g->stage_bytes < bytes) {
     if (g->stage) (void)hipFree(g->stage);
@@ -563,39 +577,48 @@ static int stage_for(jfc_graph *g, size_t bytes, void **p) {
   *p = g->stage;
   return JFC_OK;
 }
+static int sync_user(jfc_graph *g) {
+  JFC_HIP(hipStreamSynchronize(g->ustream));
+  return JFC_OK;
+}
 static int put_rows(jfc_graph *g, const InstDesc &I, void *dst, const void *src, size_t elem,
                     int on_device) {
   const size_t row = (size_t)I.nx * elem, rows = (size_t)I.ny * I.nz;
+  hipStream_t st = g->ustream;
   if (I.px == I.nx) {
-    JFC_HIP(hipMemcpy(dst, src, row * rows,
-                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpyAsync(dst, src, row * rows,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     return JFC_OK;
   }
   if (!on_device) {
     void *tmp = nullptr;
     if (int rc = stage_for(g, row * rows, &tmp)) return rc;
-    JFC_HIP(hipMemcpy(tmp, src, row * rows, hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpyAsync(tmp, src, row * rows, hipMemcpyHostToDevice, st));
     src = tmp;
   }
-  JFC_HIP(hipMemcpy2D(dst, (size_t)I.px * elem, src, row, row, rows, hipMemcpyDeviceToDevice));
+  JFC_HIP(hipMemcpy2DAsync(dst, (size_t)I.px * elem, src, row, row, rows,
+                           hipMemcpyDeviceToDevice, st));
   return JFC_OK;
 }
 static int get_rows(jfc_graph *g, const InstDesc &I, void *dst, const void *src, size_t elem,
                     int on_device) {
   const size_t row = (size_t)I.nx * elem, rows = (size_t)I.ny * I.nz;
+  hipStream_t st = g->ustream;
   if (I.px == I.nx) {
-    JFC_HIP(hipMemcpy(dst, src, row * rows,
-                      on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    JFC_HIP(hipMemcpyAsync(dst, src, row * rows,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     return JFC_OK;
   }
   if (on_device) {
-    JFC_HIP(hipMemcpy2D(dst, row, src, (size_t)I.px * elem, row, rows, hipMemcpyDeviceToDevice));
+    JFC_HIP(hipMemcpy2DAsync(dst, row, src, (size_t)I.px * elem, row, rows,
+                             hipMemcpyDeviceToDevice, st));
     return JFC_OK;
   }
   void *tmp = nullptr;
   if (int rc = stage_for(g, row * rows, &tmp)) return rc;
-  JFC_HIP(hipMemcpy2D(tmp, row, src, (size_t)I.px * elem, row, rows, hipMemcpyDeviceToDevice));
-  JFC_HIP(hipMemcpy(dst, tmp, row * rows, hipMemcpyDeviceToHost));
+  JFC_HIP(hipMemcpy2DAsync(tmp, row, src, (size_t)I.px * elem, row, rows,
+                           hipMemcpyDeviceToDevice, st));
+  JFC_HIP(hipMemcpyAsync(dst, tmp, row * rows, hipMemcpyDeviceToHost, st));
   return JFC_OK;
 }
 static int put_plane(jfc_graph *g, const InstDesc &I, int *dst, const int32_t *src,
@@ -603,15 +626,38 @@ static int put_plane(jfc_graph *g, const InstDesc &I, int *dst, const int32_t *s
   return put_rows(g, I, dst, src, sizeof(int32_t), on_device);
 }
 
-static int validate(jfc_graph *g, const int *p, long long n, int lo, int hi,
-                    const char *what) {
-  launch_fill_i32(g->d_flagword + 1, 1, 0, 0);
-  launch_validate(p, n, lo, hi, g->d_flagword + 1, 0);
-  int bad = 0;
-  JFC_HIP(hipMemcpy(&bad, g->d_flagword + 1, sizeof(int), hipMemcpyDeviceToHost));
+// Range check of up to two planes of n stored values on the graph's stream;
+// waits for the stream.  Returns JFC_ERR_OVERFLOW naming the first plane
+// with a value outside [lo, hi].
+static int validate(jfc_graph *g, const int *p0, const int *p1, long long n, int lo, int hi,
+                    const char *what0, const char *what1) {
+  if (int rc = g->ensure_pin()) return rc;
+  hipStream_t st = g->ustream;
+  launch_fill_i32(g->d_flagword + 1, 2, 0, st);
+  launch_validate(p0, n, lo, hi, g->d_flagword + 1, st);
+  if (p1) launch_validate(p1, n, lo, hi, g->d_flagword + 2, st);
+  JFC_HIP(hipMemcpyAsync(&g->pin->bad, g->d_flagword + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  JFC_HIP(hipMemcpyAsync(&g->pin->flag, g->d_flagword + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (int rc = sync_user(g)) return rc;
+  const char *bad = g->pin->bad ? what0 : (p1 && g->pin->flag) ? what1 : nullptr;
   if (bad)
-    return fail(JFC_ERR_OVERFLOW, std::string(what) + " outside [" + std::to_string(lo) +
-                                      ", " + std::to_string(hi) + "]");
+    return fail(JFC_ERR_OVERFLOW, std::string(bad) + " outside [" + std::to_string(lo) + ", " +
+                                      std::to_string(hi) + "]");
+  return JFC_OK;
+}
+
+// A setter that failed leaves the plane(s) it wrote zeroed and the instance
+// marked incomplete: the solve refuses it until the inputs are set again.
+static void input_failed(jfc_graph *g, int inst, int bits) {
+  g->caps_set[inst] &= ~bits;
+  g->caps_failed[inst] |= bits;
+  g->mask_dirty = true;
+  g->solved = false;
+}
+
+extern "C" int jfc_set_stream(jfc_graph *g, void *stream) {
+  if (!g) return fail(JFC_ERR_INVALID, "null graph");
+  g->ustream = (hipStream_t)stream;
   return JFC_OK;
 }
 
@@ -622,17 +668,19 @@ extern "C" int jfc_set_terminal_caps(jfc_graph *g, int inst, const int32_t *cap_
   if (!cap_source || !cap_sink) return fail(JFC_ERR_INVALID, "null capacity pointer");
   const long long n = g->inst_stored(inst);
   const long long o = g->inst[inst].node_off;
-  if ((rc = put_plane(g, g->inst[inst], g->cs + o, cap_source, on_device)) ||
-      (rc = put_plane(g, g->inst[inst], g->ct + o, cap_sink, on_device)))
-    return rc;
-  if ((rc = validate(g, g->cs + o, n, 0, JFC_MAX_TERMINAL_CAP, "source capacity")) ||
-      (rc = validate(g, g->ct + o, n, 0, JFC_MAX_TERMINAL_CAP, "sink capacity"))) {
-    launch_fill_i32(g->cs + o, n, 0, 0);
-    launch_fill_i32(g->ct + o, n, 0, 0);
-    (void)hipDeviceSynchronize();
+  if (!(rc = put_plane(g, g->inst[inst], g->cs + o, cap_source, on_device)) &&
+      !(rc = put_plane(g, g->inst[inst], g->ct + o, cap_sink, on_device)))
+    rc = validate(g, g->cs + o, g->ct + o, n, 0, JFC_MAX_TERMINAL_CAP, "source capacity",
+                  "sink capacity");
+  if (rc) {
+    launch_fill_i32(g->cs + o, n, 0, g->ustream);
+    launch_fill_i32(g->ct + o, n, 0, g->ustream);
+    (void)hipStreamSynchronize(g->ustream);
+    input_failed(g, inst, 1);
     return rc;
   }
   g->caps_set[inst] |= 1;
+  g->caps_failed[inst] &= ~1;
   g->solved = false;
   return JFC_OK;
 }
@@ -643,18 +691,25 @@ extern "C" int jfc_set_excess(jfc_graph *g, int inst, const int32_t *ef, int on_
   if (!ef) return fail(JFC_ERR_INVALID, "null excess pointer");
   const long long n = g->inst_stored(inst);
   const long long o = g->inst[inst].node_off;
-  int *tmp = nullptr;
-  if ((rc = dalloc(&tmp, n, "excess staging"))) return rc;
-  launch_fill_i32(tmp, n, 0, 0);
-  rc = put_plane(g, g->inst[inst], tmp, ef, on_device);
-  if (!rc) rc = validate(g, tmp, n, -JFC_MAX_TERMINAL_CAP, JFC_MAX_TERMINAL_CAP, "excess");
+  // staged in the instance's slice of the excess plane, which every solve
+  // rewrites from cs - ct before reading it
+  rc = put_plane(g, g->inst[inst], g->e + o, ef, on_device);
+  if (!rc)
+    rc = validate(g, g->e + o, nullptr, n, -JFC_MAX_TERMINAL_CAP, JFC_MAX_TERMINAL_CAP,
+                  "excess", nullptr);
   if (!rc) {
-    launch_excess_split(tmp, g->cs + o, g->ct + o, n, 0);
-    if (hipDeviceSynchronize() != hipSuccess) rc = fail(JFC_ERR_DEVICE, "excess split failed");
+    launch_excess_split(g->e + o, g->cs + o, g->ct + o, n, g->ustream);
+    rc = sync_user(g);
   }
-  dfree(tmp);
-  if (rc) return rc;
+  if (rc) {
+    launch_fill_i32(g->cs + o, n, 0, g->ustream);
+    launch_fill_i32(g->ct + o, n, 0, g->ustream);
+    (void)hipStreamSynchronize(g->ustream);
+    input_failed(g, inst, 1);
+    return rc;
+  }
   g->caps_set[inst] |= 1;
+  g->caps_failed[inst] &= ~1;
   g->solved = false;
   return JFC_OK;
 }
@@ -668,17 +723,22 @@ extern "C" int jfc_set_neighbor_caps(jfc_graph *g, int inst, int arc, const int3
     return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
   const long long n = g->inst_stored(inst);
   int *dst = g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off;
-  if ((rc = put_plane(g, g->inst[inst], dst, cap, on_device))) return rc;
-  if ((rc = validate(g, dst, n, 0, JFC_MAX_NEIGHBOR_CAP, "neighbour capacity"))) {
-    launch_fill_i32(dst, n, 0, 0);
-    (void)hipDeviceSynchronize();
-    return rc;
-  }
-  launch_zero_border(g->shape, g->inst[inst], g->c0 + (long long)arc * g->Ntot, arc, 0);
-  JFC_HIP(hipDeviceSynchronize());
-  g->caps_set[inst] |= 2 << arc;
+  rc = put_plane(g, g->inst[inst], dst, cap, on_device);
+  if (!rc) rc = validate(g, dst, nullptr, n, 0, JFC_MAX_NEIGHBOR_CAP, "neighbour capacity", nullptr);
+  // on failure the plane is zeroed and the mask of the inputs is rebuilt
+  // before the next solve, so no stale residual bit of the old arcs survives
   g->mask_dirty = true;
   g->solved = false;
+  if (rc) {
+    launch_fill_i32(dst, n, 0, g->ustream);
+    (void)hipStreamSynchronize(g->ustream);
+    input_failed(g, inst, 2 << arc);
+    return rc;
+  }
+  launch_zero_border(g->shape, g->inst[inst], g->c0 + (long long)arc * g->Ntot, arc, g->ustream);
+  if ((rc = sync_user(g))) return rc;
+  g->caps_set[inst] |= 2 << arc;
+  g->caps_failed[inst] &= ~(2 << arc);
   return JFC_OK;
 }
 
@@ -687,30 +747,45 @@ extern "C" int jfc_set_seeds(jfc_graph *g, int inst, const uint8_t *fg, const ui
   int rc;
   if ((rc = check_inst(g, inst))) return rc;
   const InstDesc &I = g->inst[inst];
-  const long long n = g->inst_stored(inst);
   const long long o = I.node_off;
-  // seeds staged in the stored row layout (pad entries unseeded)
-  const unsigned char *dfg = nullptr, *dbg = nullptr;
-  unsigned char *tfg = nullptr, *tbg = nullptr;
-  if (fg) {
-    if ((rc = dalloc(&tfg, n, "seed staging"))) return rc;
-    launch_fill_u8(tfg, n, 0, 0);
-    if ((rc = put_rows(g, I, tfg, fg, 1, on_device))) { dfree(tfg); return rc; }
-    dfg = tfg;
+  const long long rows = (long long)I.ny * I.nz, n = rows * I.nx;
+  // the kernel reads the caller's unpadded seed bytes; host bytes are staged
+  // in one contiguous copy each
+  const unsigned char *dfg = fg, *dbg = bg;
+  if (!on_device && (fg || bg)) {
+    void *tmp = nullptr;
+    if ((rc = stage_for(g, 2 * (size_t)n, &tmp))) return rc;
+    unsigned char *s = (unsigned char *)tmp;
+    if (fg) JFC_HIP(hipMemcpyAsync(s, fg, n, hipMemcpyHostToDevice, g->ustream));
+    if (bg) JFC_HIP(hipMemcpyAsync(s + n, bg, n, hipMemcpyHostToDevice, g->ustream));
+    dfg = fg ? s : nullptr;
+    dbg = bg ? s + n : nullptr;
   }
-  if (bg) {
-    if ((rc = dalloc(&tbg, n, "seed staging"))) { dfree(tfg); return rc; }
-    launch_fill_u8(tbg, n, 0, 0);
-    if ((rc = put_rows(g, I, tbg, bg, 1, on_device))) { dfree(tfg); dfree(tbg); return rc; }
-    dbg = tbg;
-  }
-  launch_seeds(g->cs + o, g->ct + o, dfg, dbg, n, 0);
-  hipError_t err = hipDeviceSynchronize();
-  dfree(tfg);
-  dfree(tbg);
-  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
+  launch_seeds(g->cs + o, g->ct + o, dfg, dbg, I.nx, I.px, rows, g->ustream);
+  if ((rc = sync_user(g))) return rc;
   g->caps_set[inst] |= 1;
   g->solved = false;
+  return JFC_OK;
+}
+
+extern "C" int jfc_input_planes(jfc_graph *g, int inst, int32_t **cap_source,
+                                int32_t **cap_sink, int32_t **neighbor, int64_t *row_pitch,
+                                int64_t *plane_stride) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  const long long o = g->inst[inst].node_off;
+  if (cap_source) *cap_source = g->cs + o;
+  if (cap_sink) *cap_sink = g->ct + o;
+  if (neighbor) *neighbor = g->c0 + o;
+  if (row_pitch) *row_pitch = g->inst[inst].px;
+  if (plane_stride) *plane_stride = g->Ntot;
+  return JFC_OK;
+}
+
+extern "C" int jfc_inputs_written(jfc_graph *g, int inst) {
+  int rc;
+  if ((rc = check_inst(g, inst))) return rc;
+  graph_mark_inputs(g, inst);
   return JFC_OK;
 }
 
@@ -860,9 +935,12 @@ struct Solver {
     return JFC_OK;
   }
 
+  int cuts = 0;  // canonical-cut searches run
+
   // Returns 1 in *found when an augmenting path still exists.
   int canonical_cut(int *found) {
     hipEvent_t a = tm.mark();
+    ++cuts;
     launch_fill_i32(g->d_flagword, 1, 0, st);
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, st);
     int rc = search(kCut, &s.cut_passes);
@@ -970,10 +1048,14 @@ struct Solver {
     g->slot = 0;
     // the residual-arc masks restart from the inputs' with every plane
     // marked fresh: k_push reads fresh planes from c0 and writes them to cap
+    const long long K = g->Kg, nodes = g->Nuser;
     if (g->mask_dirty) {
       launch_build_mask(g->shape, T, P, g->mask0, st);
       g->mask_dirty = false;
+      s.inputs_fresh = 1;
+      s.solve_bytes += nodes * (4 * K + 4);  // read K planes, write mask0
     }
+    s.solve_bytes += nodes * (8 + 12);  // mask0 -> rmask copy; fold: read cs, ct, write e
     JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
                            hipMemcpyDeviceToDevice, st));
     // Fast-Cut reads and writes the residual planes directly
@@ -1089,6 +1171,15 @@ struct Solver {
     s.relabel_tiles = totals[1] + (long long)g->ntiles * s.global_relabels;
     s.cut_tiles = totals[2];
     s.push_bytes = totals[0] * 256LL * push_bytes_per_node(g->Kg);
+    // relabels: a first pass over every tile (2D: read e and mask, write h;
+    // 3D: after a pass over the nodes that reads e and writes h), later
+    // passes read h, e and mask and write h; cut searches: one pass over the
+    // nodes (read e, write the label) and flood visits (read label, e and
+    // mask, write label)
+    const long long first = (long long)g->ntiles * s.global_relabels;
+    s.solve_bytes += s.push_bytes + first * 256LL * (g->dim == 3 ? 16 : 12) +
+                     totals[1] * 256LL * 16 + (g->dim == 3 ? 8LL * nodes * s.global_relabels : 0) +
+                     5LL * nodes * cuts + totals[2] * 256LL * 10;
     s.ms_total = Timer::ms(t0, t1);
     for (auto &p : push_ev) s.ms_push += Timer::ms(p.first, p.second);
     for (auto &p : gr_ev) s.ms_relabel += Timer::ms(p.first, p.second);
@@ -1126,8 +1217,14 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (!sh) return fail(JFC_ERR_INVALID, "tile shape does not match the graph's dimensionality");
   int rc = g->set_shape(sh);
   if (rc) return rc;
+  for (size_t i = 0; i < g->inst.size(); ++i)
+    if (g->caps_failed[i])
+      return fail(JFC_ERR_STATE, "instance " + std::to_string(i) +
+                                     ": an input setter failed and zeroed its plane; set the "
+                                     "capacities again before solving");
   std::memset(&S.s, 0, sizeof(S.s));
-  S.st = (hipStream_t)S.o.stream;
+  // the options' stream, else the graph's stream (jfc_set_stream)
+  S.st = S.o.stream ? (hipStream_t)S.o.stream : g->ustream;
   if (!S.st) {
     // the legacy default stream cannot be captured: solve on a stream of the
     // graph's own, ordered after the work already queued on the default one
@@ -1175,7 +1272,7 @@ extern "C" int jfc_get_labels(jfc_graph *g, int inst, uint8_t *labels, int on_de
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   if ((rc = get_rows(g, g->inst[inst], labels, g->lab + g->inst[inst].node_off, 1, on_device)))
     return rc;
-  return JFC_OK;
+  return sync_user(g);
 }
 
 extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *tags,
@@ -1186,17 +1283,18 @@ extern "C" int jfc_get_tags(jfc_graph *g, int inst, const uint8_t *fg, uint8_t *
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   if (!g->tag && (rc = dalloc(&g->tag, (size_t)g->Ntot, "tag plane"))) return rc;
   const long long o = g->inst[inst].node_off;
-  launch_fill_u8(g->tag, g->Ntot, 0, 0);
+  launch_fill_u8(g->tag, g->Ntot, 0, g->ustream);
   if ((rc = put_rows(g, g->inst[inst], g->tag + o, fg, 1, on_device))) return rc;
   Solver S{};
   S.g = g;
   jfc_default_options(&S.o);
-  S.st = 0;
+  S.st = g->ustream;
   S.tm.pool = &g->ev_pool;
-  S.tm.st = 0;
+  S.tm.st = g->ustream;
   int passes = 0;
   if ((rc = S.tag_pass(&passes))) return rc;
-  return get_rows(g, g->inst[inst], tags, g->tag + o, 1, on_device);
+  if ((rc = get_rows(g, g->inst[inst], tags, g->tag + o, 1, on_device))) return rc;
+  return sync_user(g);
 }
 
 extern "C" int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy) {
@@ -1204,14 +1302,14 @@ extern "C" int jfc_get_energy(jfc_graph *g, int inst, int64_t *energy) {
   if ((rc = check_inst(g, inst))) return rc;
   if (!energy) return fail(JFC_ERR_INVALID, "null output");
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
+  if ((rc = g->ensure_pin())) return rc;
   const int nI = (int)g->inst.size();
-  std::vector<long long> red((size_t)nI * RED_SLOTS);
-  JFC_HIP(hipMemcpy(red.data(), g->d_red, red.size() * sizeof(long long), hipMemcpyDeviceToHost));
-  for (int i = 0; i < nI; ++i) red[i * RED_SLOTS + RED_ENERGY] = 0;
-  JFC_HIP(hipMemcpy(g->d_red, red.data(), red.size() * sizeof(long long), hipMemcpyHostToDevice));
-  launch_energy(g->shape, g->tiles(), g->planes(), g->d_red, 0);
-  JFC_HIP(hipMemcpy(red.data(), g->d_red, red.size() * sizeof(long long), hipMemcpyDeviceToHost));
-  *energy = red[inst * RED_SLOTS + RED_ENERGY];
+  launch_fill_i32((int *)g->d_energy, 2LL * nI, 0, g->ustream);
+  launch_energy(g->shape, g->tiles(), g->planes(), g->d_energy, g->ustream);
+  JFC_HIP(hipMemcpyAsync(&g->pin->red[0], g->d_energy + inst, sizeof(long long),
+                         hipMemcpyDeviceToHost, g->ustream));
+  if ((rc = sync_user(g))) return rc;
+  *energy = g->pin->red[0];
   return JFC_OK;
 }
 
@@ -1222,7 +1320,7 @@ extern "C" int jfc_get_terminal_caps(jfc_graph *g, int inst, int32_t *cap_source
   const InstDesc &I = g->inst[inst];
   if (cap_source && (rc = get_rows(g, I, cap_source, g->cs + I.node_off, 4, on_device))) return rc;
   if (cap_sink && (rc = get_rows(g, I, cap_sink, g->ct + I.node_off, 4, on_device))) return rc;
-  return JFC_OK;
+  return sync_user(g);
 }
 
 extern "C" int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *cap,
@@ -1232,8 +1330,10 @@ extern "C" int jfc_get_neighbor_caps(jfc_graph *g, int inst, int arc, int32_t *c
   if (!cap) return fail(JFC_ERR_INVALID, "null output");
   if (arc < 0 || arc >= g->inst[inst].K)
     return fail(JFC_ERR_INVALID, "arc index outside the instance's connectivity");
-  return get_rows(g, g->inst[inst], cap, g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off,
-                  4, on_device);
+  if ((rc = get_rows(g, g->inst[inst], cap,
+                     g->c0 + (long long)arc * g->Ntot + g->inst[inst].node_off, 4, on_device)))
+    return rc;
+  return sync_user(g);
 }
 
 extern "C" int jfc_debug_tiles(jfc_graph *g, int inst, int32_t *ntiles, int32_t *tile_dims,
@@ -1363,34 +1463,32 @@ extern "C" int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax)
   return JFC_OK;
 }
 
+// Gradients into device outputs directly, or into host outputs through the
+// staging buffer (n floats per terminal output, K*n for the neighbour planes).
 extern "C" int jfc_backward_all(jfc_graph *g, float *d_cap_source, float *d_cap_sink,
                                 float *d_neighbor, int on_device) {
   if (!g) return fail(JFC_ERR_INVALID, "null graph");
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   const long long n = g->Nuser;
+  hipStream_t st = g->ustream;
   if (on_device) {
     launch_backward_all(g->shape, g->tiles(), g->planes(), d_cap_source, d_cap_sink,
-                        d_neighbor, n, 0);
-    JFC_HIP(hipDeviceSynchronize());
-    return JFC_OK;
+                        d_neighbor, n, st);
+    return sync_user(g);
   }
   int rc;
-  float *a = nullptr, *b = nullptr, *c = nullptr;
-  if ((d_cap_source && (rc = dalloc(&a, n, "gradient"))) ||
-      (d_cap_sink && (rc = dalloc(&b, n, "gradient"))) ||
-      (d_neighbor && (rc = dalloc(&c, n * g->Kg, "gradient")))) {
-    dfree(a); dfree(b); dfree(c);
-    return rc;
-  }
-  if (c) JFC_HIP(hipMemcpy(c, d_neighbor, n * g->Kg * 4, hipMemcpyHostToDevice));
-  launch_backward_all(g->shape, g->tiles(), g->planes(), a, b, c, n, 0);
-  hipError_t err = hipDeviceSynchronize();
-  if (err == hipSuccess && a) err = hipMemcpy(d_cap_source, a, n * 4, hipMemcpyDeviceToHost);
-  if (err == hipSuccess && b) err = hipMemcpy(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost);
-  if (err == hipSuccess && c) err = hipMemcpy(d_neighbor, c, n * g->Kg * 4, hipMemcpyDeviceToHost);
-  dfree(a); dfree(b); dfree(c);
-  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
-  return JFC_OK;
+  void *tmp = nullptr;
+  if ((rc = stage_for(g, (size_t)n * (2 + g->Kg) * sizeof(float), &tmp))) return rc;
+  float *a = d_cap_source ? (float *)tmp : nullptr;
+  float *b = d_cap_sink ? (float *)tmp + n : nullptr;
+  float *c = d_neighbor ? (float *)tmp + 2 * n : nullptr;
+  // planes beyond an instance's connectivity keep the caller's values
+  if (c) JFC_HIP(hipMemcpyAsync(c, d_neighbor, n * g->Kg * 4, hipMemcpyHostToDevice, st));
+  launch_backward_all(g->shape, g->tiles(), g->planes(), a, b, c, n, st);
+  if (a) JFC_HIP(hipMemcpyAsync(d_cap_source, a, n * 4, hipMemcpyDeviceToHost, st));
+  if (b) JFC_HIP(hipMemcpyAsync(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost, st));
+  if (c) JFC_HIP(hipMemcpyAsync(d_neighbor, c, n * g->Kg * 4, hipMemcpyDeviceToHost, st));
+  return sync_user(g);
 }
 
 extern "C" int jfc_debug_sweep_counts(jfc_graph *g, int32_t *counts, int32_t n) {
@@ -1407,25 +1505,20 @@ extern "C" int jfc_backward(jfc_graph *g, int inst, float *d_cap_source, float *
   if (!g->solved) return fail(JFC_ERR_STATE, "graph not solved");
   const InstDesc &I = g->inst[inst];
   const long long n = g->inst_nodes(inst);
+  hipStream_t st = g->ustream;
   if (on_device) {
     launch_backward(g->shape, g->tiles(), g->planes(), I, d_cap_source, d_cap_sink,
-                    d_neighbor, 0);
-    JFC_HIP(hipDeviceSynchronize());
-    return JFC_OK;
+                    d_neighbor, st);
+    return sync_user(g);
   }
-  float *a = nullptr, *b = nullptr, *c = nullptr;
-  if ((d_cap_source && (rc = dalloc(&a, n, "gradient"))) ||
-      (d_cap_sink && (rc = dalloc(&b, n, "gradient"))) ||
-      (d_neighbor && (rc = dalloc(&c, n * I.K, "gradient")))) {
-    dfree(a); dfree(b); dfree(c);
-    return rc;
-  }
-  launch_backward(g->shape, g->tiles(), g->planes(), I, a, b, c, 0);
-  hipError_t err = hipDeviceSynchronize();
-  if (err == hipSuccess && a) err = hipMemcpy(d_cap_source, a, n * 4, hipMemcpyDeviceToHost);
-  if (err == hipSuccess && b) err = hipMemcpy(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost);
-  if (err == hipSuccess && c) err = hipMemcpy(d_neighbor, c, n * I.K * 4, hipMemcpyDeviceToHost);
-  dfree(a); dfree(b); dfree(c);
-  if (err != hipSuccess) return fail(JFC_ERR_DEVICE, hipGetErrorString(err));
-  return JFC_OK;
+  void *tmp = nullptr;
+  if ((rc = stage_for(g, (size_t)n * (2 + I.K) * sizeof(float), &tmp))) return rc;
+  float *a = d_cap_source ? (float *)tmp : nullptr;
+  float *b = d_cap_sink ? (float *)tmp + n : nullptr;
+  float *c = d_neighbor ? (float *)tmp + 2 * n : nullptr;
+  launch_backward(g->shape, g->tiles(), g->planes(), I, a, b, c, st);
+  if (a) JFC_HIP(hipMemcpyAsync(d_cap_source, a, n * 4, hipMemcpyDeviceToHost, st));
+  if (b) JFC_HIP(hipMemcpyAsync(d_cap_sink, b, n * 4, hipMemcpyDeviceToHost, st));
+  if (c) JFC_HIP(hipMemcpyAsync(d_neighbor, c, n * I.K * 4, hipMemcpyDeviceToHost, st));
+  return sync_user(g);
 }

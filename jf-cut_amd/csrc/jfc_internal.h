@@ -129,8 +129,9 @@ void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
                         hipStream_t st);
-void launch_seeds(int *cs, int *ct, const unsigned char *fg,
-                  const unsigned char *bg, long long n, hipStream_t st);
+// fg / bg: rows of nx bytes (the caller's numbering); cs / ct: rows of px.
+void launch_seeds(int *cs, int *ct, const unsigned char *fg, const unsigned char *bg, int nx,
+                  int px, long long rows, hipStream_t st);
 void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
                          hipStream_t st);
 
@@ -199,7 +200,8 @@ void launch_tag_init(int shape, const Tiles &T, unsigned char *tag,
 void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
                 const Queue &q, unsigned char *dirty, const unsigned char *object,
                 hipStream_t st);
-void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
+// energy: one int64 per instance, zeroed by the caller.
+void launch_energy(int shape, const Tiles &T, const Planes &P, long long *energy,
                    hipStream_t st);
 void launch_backward(int shape, const Tiles &T, const Planes &P,
                      const InstDesc &I, float *dcs, float *dct, float *dnb,

@@ -264,22 +264,43 @@ void graph_mark_inputs(jfc_graph *g, int inst);
     }                                                                           \
   } while (0)
 
-extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
-                               int bpc, const uint8_t *seeds, int on_device,
-                               const jfc_model_params *prm, jfc_model_stats *stats) {
-  InstDesc I;
-  int *cs = nullptr, *ct = nullptr, *c0 = nullptr;
-  long long stride = 0;
-  int rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride);
-  if (rc) return rc;
-  if (!image || !seeds || !prm) return model_fail(JFC_ERR_INVALID, "null argument");
+// Parameter checks that need no graph.  The unary of a node is
+// round(scale * d2 / den) with d2 <= C * (maxval * 2^KM_FRAC)^2 (a feature and
+// a centroid both in [0, maxval]) and den = range^2 * 2^(2 KM_FRAC): the
+// product scale * d2 must fit int64 and the unary must not exceed the
+// terminal-capacity limit, or the solve's int32 arithmetic would not be exact.
+static int check_model_params(int channels, int bpc, const jfc_model_params *prm) {
+  if (!prm) return model_fail(JFC_ERR_INVALID, "null argument");
   if (!((channels == 1 || channels == 3) && (bpc == 1 || (bpc == 2 && channels == 1))))
     return model_fail(JFC_ERR_INVALID, "image must be 1 or 3 channels of uint8, or 1 channel of uint16");
   if (prm->k_fg < 1 || prm->k_fg > KMAX_CLUSTERS || prm->k_bg < 1 || prm->k_bg > KMAX_CLUSTERS)
     return model_fail(JFC_ERR_INVALID, "cluster count must be 1..16");
-  if (prm->range <= 0 || prm->unary_scale < 0 || !(prm->lambda_xy >= 0) ||
-      !(prm->lambda_z >= 0) || prm->lambda_xy > 255 || prm->lambda_z > 255)
+  if (prm->range <= 0 || prm->range > (1 << 20) || prm->unary_scale < 0 ||
+      !(prm->lambda_xy >= 0) || !(prm->lambda_z >= 0) || prm->lambda_xy > 255 ||
+      prm->lambda_z > 255)
     return model_fail(JFC_ERR_INVALID, "invalid model parameters");
+  const long double maxval = bpc == 1 ? 255.0L : 65535.0L;
+  const long double d2max = (long double)channels * (maxval * (1 << KM_FRAC)) * (maxval * (1 << KM_FRAC));
+  const long double den = (long double)prm->range * prm->range * (long double)(1LL << (2 * KM_FRAC));
+  const long double prod = (long double)prm->unary_scale * d2max + den;
+  if (prod >= 9.0e18L || (long double)prm->unary_scale * d2max / den + 0.5L > (long double)JFC_MAX_TERMINAL_CAP)
+    return model_fail(JFC_ERR_OVERFLOW,
+                      "unary_scale too large for the image range: a unary could exceed " +
+                          std::to_string(JFC_MAX_TERMINAL_CAP));
+  return JFC_OK;
+}
+
+extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
+                               int bpc, const uint8_t *seeds, int on_device,
+                               const jfc_model_params *prm, jfc_model_stats *stats) {
+  int rc = check_model_params(channels, bpc, prm);
+  if (rc) return rc;
+  InstDesc I;
+  int *cs = nullptr, *ct = nullptr, *c0 = nullptr;
+  long long stride = 0;
+  rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride);
+  if (rc) return rc;
+  if (!image || !seeds) return model_fail(JFC_ERR_INVALID, "null argument");
   const long long n = (long long)I.nx * I.ny * I.nz;
   if (n >= (1LL << 31)) return model_fail(JFC_ERR_INVALID, "instance too large for the model path");
   const int C = channels;

@@ -96,18 +96,23 @@ void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
   if (n > 0) k_validate<<<grid_for(n), 256, 0, st>>>(p, n, lo, hi, bad);
 }
 
-__global__ void k_seeds(int *cs, int *ct, const unsigned char *fg,
-                        const unsigned char *bg, long long n) {
+// Hard seeds from the caller's unpadded seed bytes (row r of nx bytes) onto
+// the stored planes (row r at r * px): fg wins where both are set, the rule
+// of graphcut_compute_excessflow.cl:71-74.
+__global__ void k_seeds(int *cs, int *ct, const unsigned char *fg, const unsigned char *bg,
+                        int nx, int px, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    if (fg && fg[i]) { cs[i] = 1 << 30; ct[i] = 0; }
-    else if (bg && bg[i]) { cs[i] = 0; ct[i] = 1 << 30; }
+    const long long v = (i / nx) * px + i % nx;
+    if (fg && fg[i]) { cs[v] = 1 << 30; ct[v] = 0; }
+    else if (bg && bg[i]) { cs[v] = 0; ct[v] = 1 << 30; }
   }
 }
 
-void launch_seeds(int *cs, int *ct, const unsigned char *fg,
-                  const unsigned char *bg, long long n, hipStream_t st) {
-  if (n > 0) k_seeds<<<grid_for(n), 256, 0, st>>>(cs, ct, fg, bg, n);
+void launch_seeds(int *cs, int *ct, const unsigned char *fg, const unsigned char *bg, int nx,
+                  int px, long long rows, hipStream_t st) {
+  const long long n = rows * nx;
+  if (n > 0 && (fg || bg)) k_seeds<<<grid_for(n), 256, 0, st>>>(cs, ct, fg, bg, nx, px, n);
 }
 
 __global__ void k_excess_split(const int *ef, int *cs, int *ct, long long n) {
@@ -570,9 +575,10 @@ void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mas
   JFC_DISPATCH(shape, (k_build_mask<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, mask0)));
 }
 
-// Cut energy on the input capacities (QGCWidget.cpp:2452-2520).
+// Cut energy on the input capacities (QGCWidget.cpp:2452-2520), summed per
+// instance into energy[inst] (zeroed by the caller).
 template <class S>
-__global__ __launch_bounds__(256) void k_energy(Tiles T, Planes P, long long *red) {
+__global__ __launch_bounds__(256) void k_energy(Tiles T, Planes P, long long *energy) {
   __shared__ long long buf[4];
   const Lane<S> L(threadIdx.x);
   const Chunk C(T.ntiles);
@@ -583,8 +589,7 @@ __global__ __launch_bounds__(256) void k_energy(Tiles T, Planes P, long long *re
     if (inst != cur) {
       if (cur >= 0) {
         const long long A = block_sum_ll(a, buf);
-        if (threadIdx.x == 0)
-          atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_ENERGY], (unsigned long long)A);
+        if (threadIdx.x == 0) atomicAdd((unsigned long long *)&energy[cur], (unsigned long long)A);
       }
       cur = inst;
       a = 0;
@@ -612,14 +617,13 @@ __global__ __launch_bounds__(256) void k_energy(Tiles T, Planes P, long long *re
   }
   if (cur >= 0) {
     const long long A = block_sum_ll(a, buf);
-    if (threadIdx.x == 0)
-      atomicAdd((unsigned long long *)&red[cur * RED_SLOTS + RED_ENERGY], (unsigned long long)A);
+    if (threadIdx.x == 0) atomicAdd((unsigned long long *)&energy[cur], (unsigned long long)A);
   }
 }
 
-void launch_energy(int shape, const Tiles &T, const Planes &P, long long *red,
+void launch_energy(int shape, const Tiles &T, const Planes &P, long long *energy,
                    hipStream_t st) {
-  JFC_DISPATCH(shape, (k_energy<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, red)));
+  JFC_DISPATCH(shape, (k_energy<S><<<tile_grid(T.ntiles), 256, 0, st>>>(T, P, energy)));
 }
 
 // Subgradient of the cut energy for instance `inst` (outputs in the

@@ -50,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("relabel_tiles", ctypes.c_int64), ("cut_tiles", ctypes.c_int64),
                 ("push_bytes", ctypes.c_int64), ("ms_total", ctypes.c_double),
                 ("ms_push", ctypes.c_double), ("ms_relabel", ctypes.c_double),
-                ("ms_cut", ctypes.c_double), ("local_iters", ctypes.c_int64)]
+                ("ms_cut", ctypes.c_double), ("local_iters", ctypes.c_int64),
+                ("solve_bytes", ctypes.c_int64), ("inputs_fresh", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -100,7 +101,7 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
            "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_get_initial_relabel", "jfc_default_widget_model_params",
-           "jfc_model_widget"]
+           "jfc_model_widget", "jfc_set_stream", "jfc_input_planes", "jfc_inputs_written"]
 
 
 def lib():
@@ -118,6 +119,10 @@ def lib():
         "jfc_create_batch": (I, [ctypes.POINTER(P), I, P]),
         "jfc_destroy": (None, [P]),
         "jfc_num_instances": (I, [P]),
+        "jfc_set_stream": (I, [P, P]),
+        "jfc_input_planes": (I, [P, I, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
+                                 ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "jfc_inputs_written": (I, [P, I]),
         "jfc_last_error": (ctypes.c_char_p, []),
         "jfc_set_terminal_caps": (I, [P, I, P, P, I]),
         "jfc_set_excess": (I, [P, I, P, I]),
@@ -176,23 +181,57 @@ def _check(rc, allow=(OK,)):
     return rc
 
 
-def _ptr(a, dtype):
-    """(pointer, on_device) of a numpy array or a GPU torch tensor."""
+_TORCH_DTYPE_NAMES = {np.dtype(np.int32): ("int32",), np.dtype(np.uint8): ("uint8",),
+                      np.dtype(np.float32): ("float32",),
+                      np.dtype(np.uint16): ("uint16", "int16")}
+
+
+def _is_tensor(a):
+    return hasattr(a, "data_ptr") and hasattr(a, "is_cuda")
+
+
+def _check_tensor(t, dtype, n, name):
+    """A GPU torch tensor the library reads or writes by pointer: it must be
+    on the GPU, of the element type the C-ABI expects, contiguous and hold
+    exactly n elements (the C side trusts the pointer for n elements)."""
+    if not t.is_cuda:
+        raise ValueError(f"{name}: a CPU tensor; pass a numpy array or a GPU tensor")
+    want = _TORCH_DTYPE_NAMES[np.dtype(dtype)]
+    got = str(t.dtype).replace("torch.", "")
+    if got not in want:
+        raise ValueError(f"{name}: dtype {got}, expected {want[0]}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if t.numel() != n:
+        raise ValueError(f"{name}: {t.numel()} elements, expected {n}")
+
+
+def _host_or_dev(a, dtype, n, name="array"):
+    """(pointer, on_device, keep-alive) of a host array or a GPU torch tensor
+    of exactly n elements of dtype.  Host arrays of another integer type are
+    converted only when every value survives the conversion."""
     if a is None:
-        return None, 0
-    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
-        return ctypes.c_void_p(a.data_ptr()), 1
-    arr = np.ascontiguousarray(a, dtype=dtype)
-    return arr, 0
-
-
-def _host_or_dev(a, dtype):
-    p, dev = _ptr(a, dtype)
-    if p is None:
         return None, 0, None
-    if dev:
-        return p, 1, a
-    return p.ctypes.data_as(ctypes.c_void_p), 0, p
+    if _is_tensor(a):
+        _check_tensor(a, dtype, n, name)
+        return ctypes.c_void_p(a.data_ptr()), 1, a
+    src = np.asarray(a)
+    arr = np.ascontiguousarray(src, dtype=dtype)
+    if src.dtype != arr.dtype and not np.array_equal(arr, src):
+        raise ValueError(f"{name}: values do not fit {np.dtype(dtype).name}")
+    if arr.size != n:
+        raise ValueError(f"{name}: {arr.size} elements, expected {n}")
+    return arr.ctypes.data_as(ctypes.c_void_p), 0, arr
+
+
+def _out_ptr(t, dtype, n, name):
+    """Pointer of a GPU output tensor (checked like an input), or None."""
+    if t is None:
+        return None
+    if not _is_tensor(t):
+        raise ValueError(f"{name}: out= takes GPU torch tensors")
+    _check_tensor(t, dtype, n, name)
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def default_options(**kw):
@@ -263,18 +302,41 @@ class GridGraph:
     def ndirs(self, inst=0):
         return self.shapes[inst][3]
 
+    def set_stream(self, stream):
+        """Issue this graph's work on `stream` (a hipStream_t as an int, e.g.
+        torch.cuda.current_stream().cuda_stream; 0 = the legacy default
+        stream)."""
+        _check(lib().jfc_set_stream(self._h, ctypes.c_void_p(int(stream) or None)))
+
+    def input_planes(self, inst=0):
+        """Device pointers of the stored input planes (zero-copy inputs):
+        dict of cs, ct, neighbor (ints), row_pitch, plane_stride."""
+        a, b, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        rp, ps = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().jfc_input_planes(self._h, inst, ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(c), ctypes.byref(rp), ctypes.byref(ps)))
+        return {"cs": a.value, "ct": b.value, "neighbor": c.value, "row_pitch": rp.value,
+                "plane_stride": ps.value}
+
+    def inputs_written(self, inst=0):
+        """Tell the library the input planes were written in place."""
+        _check(lib().jfc_inputs_written(self._h, inst))
+
     # ---- inputs
     def set_terminal_caps(self, cs, ct, inst=0):
-        p1, d1, k1 = _host_or_dev(cs, np.int32)
-        p2, d2, k2 = _host_or_dev(ct, np.int32)
+        n = self.nodes(inst)
+        p1, d1, k1 = _host_or_dev(cs, np.int32, n, "cap_source")
+        p2, d2, k2 = _host_or_dev(ct, np.int32, n, "cap_sink")
+        if d1 != d2:
+            raise ValueError("cap_source and cap_sink must both be host arrays or GPU tensors")
         _check(lib().jfc_set_terminal_caps(self._h, inst, p1, p2, d1))
 
     def set_excess(self, ef, inst=0):
-        p, d, k = _host_or_dev(ef, np.int32)
+        p, d, k = _host_or_dev(ef, np.int32, self.nodes(inst), "ef")
         _check(lib().jfc_set_excess(self._h, inst, p, d))
 
     def set_neighbor_caps(self, arc, cap, inst=0):
-        p, d, k = _host_or_dev(cap, np.int32)
+        p, d, k = _host_or_dev(cap, np.int32, self.nodes(inst), "cap")
         _check(lib().jfc_set_neighbor_caps(self._h, inst, arc, p, d))
 
     def set_all_neighbor_caps(self, caps, inst=0):
@@ -282,20 +344,26 @@ class GridGraph:
             self.set_neighbor_caps(d, caps[d], inst)
 
     def set_seeds(self, fg=None, bg=None, inst=0):
-        p1, d1, k1 = _host_or_dev(fg, np.uint8)
-        p2, d2, k2 = _host_or_dev(bg, np.uint8)
+        n = self.nodes(inst)
+        p1, d1, k1 = _host_or_dev(fg, np.uint8, n, "fg")
+        p2, d2, k2 = _host_or_dev(bg, np.uint8, n, "bg")
+        if fg is not None and bg is not None and d1 != d2:
+            raise ValueError("fg and bg must both be host arrays or GPU tensors")
         _check(lib().jfc_set_seeds(self._h, inst, p1, p2, max(d1, d2)))
 
     def model_image(self, image, seeds, inst=0, k_fg=5, k_bg=5, kmeans_iters=32,
                     unary_scale=100, rng=255, lambda_xy=50.0, lambda_z=None, z_separate=0):
-        img_dtype = np.uint16 if getattr(image, "dtype", None) in (np.uint16,) else np.uint8
-        if hasattr(image, "element_size"):
+        if _is_tensor(image):
             bpc = image.element_size()
         else:
-            bpc = np.dtype(img_dtype).itemsize
+            bpc = np.asarray(image).dtype.itemsize
+        img_dtype = np.uint16 if bpc == 2 else np.uint8
         channels = 3 if (getattr(image, "ndim", 1) >= 3 and image.shape[-1] == 3) else 1
-        p1, d1, k1 = _host_or_dev(image, img_dtype)
-        p2, d2, k2 = _host_or_dev(seeds, np.uint8)
+        n = self.nodes(inst)
+        p1, d1, k1 = _host_or_dev(image, img_dtype, n * channels, "image")
+        p2, d2, k2 = _host_or_dev(seeds, np.uint8, n, "seeds")
+        if d1 != d2:
+            raise ValueError("image and seeds must both be host arrays or GPU tensors")
         prm = ModelParams(k_fg, k_bg, kmeans_iters, unary_scale, rng, float(lambda_xy),
                           float(lambda_xy if lambda_z is None else lambda_z), int(z_separate))
         st = ModelStats()
@@ -309,9 +377,10 @@ class GridGraph:
         arguments override jfc_widget_model_params fields (weight as 4
         floats).  Returns the stats structure."""
         prm = widget_model_params(**kw)
-        p1, d1, k1 = _host_or_dev(rgba, np.uint8)
-        p2, d2, k2 = _host_or_dev(fg, np.uint8)
-        p3, d3, k3 = _host_or_dev(bg, np.uint8)
+        n = self.nodes(inst)
+        p1, d1, k1 = _host_or_dev(rgba, np.uint8, 4 * n, "rgba")
+        p2, d2, k2 = _host_or_dev(fg, np.uint8, n, "fg")
+        p3, d3, k3 = _host_or_dev(bg, np.uint8, n, "bg")
         if len({d1, d2, d3}) != 1:
             raise ValueError("rgba, fg and bg must all be host arrays or all device tensors")
         st = WidgetModelStats()
@@ -340,8 +409,9 @@ class GridGraph:
         return out
 
     def labels(self, inst=0, out=None):
-        if out is not None and hasattr(out, "data_ptr"):
-            _check(lib().jfc_get_labels(self._h, inst, ctypes.c_void_p(out.data_ptr()), 1))
+        if out is not None:
+            p = _out_ptr(out, np.uint8, self.nodes(inst), "labels out")
+            _check(lib().jfc_get_labels(self._h, inst, p, 1))
             return out
         lab = np.zeros(self.nodes(inst), np.uint8)
         _check(lib().jfc_get_labels(self._h, inst, lab.ctypes.data_as(ctypes.c_void_p), 0))
@@ -352,9 +422,13 @@ class GridGraph:
         foreground seeds and source-side nodes axis-connected to one through
         source-side nodes.  fg: numpy array (host) or GPU tensor (then `out`
         must be a GPU uint8 tensor too)."""
-        if hasattr(fg, "data_ptr"):
-            _check(lib().jfc_get_tags(self._h, inst, ctypes.c_void_p(fg.data_ptr()),
-                                      ctypes.c_void_p(out.data_ptr()), 1))
+        if _is_tensor(fg):
+            n = self.nodes(inst)
+            _check_tensor(fg, np.uint8, n, "fg")
+            p = _out_ptr(out, np.uint8, n, "tags out")
+            if p is None:
+                raise ValueError("a GPU fg needs a GPU out tensor")
+            _check(lib().jfc_get_tags(self._h, inst, ctypes.c_void_p(fg.data_ptr()), p, 1))
             return out
         f = np.ascontiguousarray(fg, np.uint8).ravel()
         if f.size != self.nodes(inst):
@@ -401,9 +475,12 @@ class GridGraph:
         (d_source[Ntot], d_sink[Ntot], d_neighbor[Kmax, Ntot]); into GPU tensors
         when `out` is given."""
         if out is not None:
-            ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+            n, K = self.size()
             a, b, c = out
-            _check(lib().jfc_backward_all(self._h, ptr(a), ptr(b), ptr(c), 1))
+            _check(lib().jfc_backward_all(
+                self._h, _out_ptr(a, np.float32, n, "d_cap_source"),
+                _out_ptr(b, np.float32, n, "d_cap_sink"),
+                _out_ptr(c, np.float32, K * n, "d_neighbor"), 1))
             return out
         n, K = self.size()
         a = np.zeros(n, np.float32)
@@ -470,9 +547,12 @@ class GridGraph:
         """(d_source, d_sink, d_neighbor[K, N]) as fp32 numpy arrays, or into the
         given GPU tensors (d_source, d_sink, d_neighbor)."""
         if out is not None:
+            n, K = self.nodes(inst), self.ndirs(inst)
             a, b, c = out
-            ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
-            _check(lib().jfc_backward(self._h, inst, ptr(a), ptr(b), ptr(c), 1))
+            _check(lib().jfc_backward(
+                self._h, inst, _out_ptr(a, np.float32, n, "d_cap_source"),
+                _out_ptr(b, np.float32, n, "d_cap_sink"),
+                _out_ptr(c, np.float32, K * n, "d_neighbor"), 1))
             return out
         n, K = self.nodes(inst), self.ndirs(inst)
         a = np.zeros(n, np.float32)
@@ -704,22 +784,28 @@ def _autograd_function():
         @staticmethod
         def forward(ctx, cs, ct, w, shape, connectivity):
             nx, ny, nz = shape
-            g = GridGraph([(nx, ny, nz, connectivity)])
-            ics = cs.detach().round().to(torch.int32).contiguous()
-            ict = ct.detach().round().to(torch.int32).contiguous()
-            iw = w.detach().round().to(torch.int32).contiguous()
-            torch.cuda.synchronize()
+            n = nx * ny * nz
+            if cs.numel() != n or ct.numel() != n or w.numel() != connectivity * n:
+                raise ValueError("cs, ct need N values and w K*N for the grid shape")
+            g = _graph_for((nx, ny, nz, connectivity), cs.device)
+            ics = cs.detach().round().to(torch.int32).reshape(-1).contiguous()
+            ict = ct.detach().round().to(torch.int32).reshape(-1).contiguous()
+            iw = w.detach().round().to(torch.int32).reshape(connectivity, n).contiguous()
+            # the library issues its work on torch's current stream, after the
+            # kernels that produced the capacities
+            g.set_stream(torch.cuda.current_stream(cs.device).cuda_stream)
             g.set_terminal_caps(ics, ict)
             for d in range(connectivity):
                 g.set_neighbor_caps(d, iw[d])
             g.solve()
-            n = nx * ny * nz
+            if g.last_rc != OK:
+                raise JFCError(NOT_CONVERGED, "min cut did not converge within max_sweeps: "
+                                              "the energy and gradients would not be exact")
             dcs = torch.empty(n, dtype=torch.float32, device=cs.device)
             dct = torch.empty_like(dcs)
             dw = torch.empty((connectivity, n), dtype=torch.float32, device=cs.device)
             g.backward(out=(dcs, dct, dw))
             flow = g.flow()
-            g.close()
             ctx.save_for_backward(dcs, dct, dw)
             return torch.tensor(float(flow), dtype=torch.float64, device=cs.device)
 
@@ -733,10 +819,32 @@ def _autograd_function():
 
 
 _MINCUT = None
+# One graph per (grid shape, connectivity, device), reused by every forward of
+# that shape: a training loop allocates the device planes once.
+_GRAPHS = {}
+_GRAPH_CACHE_MAX = 4
+
+
+def _graph_for(shape, device):
+    key = (tuple(shape), str(device))
+    g = _GRAPHS.pop(key, None)
+    if g is None:
+        while len(_GRAPHS) >= _GRAPH_CACHE_MAX:
+            _GRAPHS.pop(next(iter(_GRAPHS))).close()
+        g = GridGraph([tuple(shape)])
+    _GRAPHS[key] = g  # most recently used last
+    return g
+
+
+def release_min_cut_graphs():
+    """Free the graphs min_cut_energy keeps for reuse."""
+    while _GRAPHS:
+        _GRAPHS.popitem()[1].close()
 
 
 def min_cut_energy(cs, ct, w, shape, connectivity):
-    """Differentiable exact min-cut energy (torch.autograd.Function)."""
+    """Differentiable exact min-cut energy (torch.autograd.Function).  Raises
+    JFCError when the solve stops at max_sweeps without the certificate."""
     global _MINCUT
     if _MINCUT is None:
         _MINCUT = _autograd_function()

@@ -128,46 +128,146 @@ def test_config4_autograd_full_size(gpu):
         assert torch.equal(tw.grad[d].reshape(nz, ny, nx).bool(), exp), d
 
 
-def test_config4_finite_differences(gpu):
-    """+-1 integer finite differences of the min-cut energy on 10^4 non-tie
-    arcs of a reduced config-4 volume (24^3, same generator and data term),
-    each E(w+-1) an exact re-solve, batched 512 instances per launch."""
-    c = gpu.CONFIGS["c4"]
-    n3 = 24
-    vol, sd = gpu.synth_volume(n3, n3, n3, nell=c["nell"], seed=5)
-    g = gpu.GridGraph([(n3, n3, n3, 6)])
-    g.model_image(vol, sd, k_fg=c["k"], k_bg=c["k"], rng=c["rng"], lambda_xy=c["lam"])
+def _c4_graph(gpu, gd):
+    g, img, sd = gpu.build_config("c4", seed=gd["seed"])
+    del img, sd
     cs, ct = g.terminal_caps()
     caps = g.neighbor_caps()
-    g.solve()
+    assert sha(cs, ct, caps) == gd["caps_sha256"]
+    return g, cs, ct, caps
+
+
+def _cut_arcs(lab, caps, shape, dirs):
+    """(d, v) of every arc with capacity >= 1 from a source-side node to a
+    sink-side node (the arcs whose gradient is 1)."""
+    nx, ny, nz = shape
+    L = lab.reshape(nz, ny, nx).astype(bool)
+    out = []
+    for d, (dx, dy, dz) in enumerate(dirs):
+        sl = lambda a, b, n: slice(max(0, -a), n - max(0, a)) if b else slice(max(0, a), n - max(0, -a) or None)
+        src = L[sl(dz, 1, nz), sl(dy, 1, ny), sl(dx, 1, nx)]
+        dst = L[sl(dz, 0, nz), sl(dy, 0, ny), sl(dx, 0, nx)]
+        m = np.zeros_like(L)
+        m[sl(dz, 1, nz), sl(dy, 1, ny), sl(dx, 1, nx)] = src & ~dst
+        v = np.flatnonzero(m.ravel() & (caps[d] >= 1))
+        out.append(np.stack([np.full(v.size, d), v], 1))
+    return np.concatenate(out)
+
+
+def test_config4_finite_differences_full_size(gpu):
+    """+-1 integer finite differences on the 512^3 volume itself: 400 arcs
+    (half of them cut arcs), each E(w +- 1) an exact re-solve of all 1.3e8
+    nodes, against the backward's cut-edge indicator; ties (E not
+    differentiable at w) are skipped and counted."""
+    torch = pytest.importorskip("torch")
+    gd = goldens().get("c4")
+    if gd is None:
+        pytest.skip("no golden for c4")
+    g, cs, ct, caps = _c4_graph(gpu, gd)
+    nx, ny, nz = gd["shape"]
+    N = nx * ny * nz
+    opts = dict(max_local_iters=4)
+    st = g.solve(**opts)
     E0 = g.flow()
+    assert E0 == gd["flow"]
     lab = g.labels()
-    _, _, grad = g.backward()
+    dev = torch.device("cuda:0")
+    dw = torch.empty((6, N), dtype=torch.float32, device=dev)
+    g.set_stream(torch.cuda.current_stream().cuda_stream)
+    g.backward(out=(None, None, dw))
+    rng = np.random.default_rng(3)
+    cut = _cut_arcs(lab, caps, (nx, ny, nz), gpu.DIRS_3D)
+    anyarc = np.stack([rng.integers(0, 6, 4000), rng.integers(0, N, 4000)], 1)
+    anyarc = anyarc[caps[anyarc[:, 0], anyarc[:, 1]] >= 1]
+    pick = np.concatenate([cut[rng.choice(len(cut), 200, replace=False)], anyarc[:200]])
+    grad = dw[torch.from_numpy(pick[:, 0]).to(dev), torch.from_numpy(pick[:, 1]).to(dev)]
+    grad = grad.cpu().numpy()
+    planes = torch.from_numpy(caps).to(dev)
+    # the setters read the perturbed planes in torch's stream order
+    g.set_stream(torch.cuda.current_stream().cuda_stream)
+    checked = ties = ones = 0
+    for i, (d, v) in enumerate(pick):
+        d, v = int(d), int(v)
+        E = []
+        for delta in (1, -1):
+            planes[d, v] += delta
+            g.set_neighbor_caps(d, planes[d])
+            g.solve(**opts)
+            E.append(g.flow())
+            planes[d, v] -= delta
+        g.set_neighbor_caps(d, planes[d])
+        up, dn = E[0] - E0, E0 - E[1]
+        if i % 50 == 0:
+            say("fd full size", i, (d, v), up, dn, float(grad[i]))
+        if up != dn:
+            ties += 1
+            continue
+        assert grad[i] == up, (d, v, up, float(grad[i]))
+        checked += 1
+        ones += up
+    say("full-size finite differences checked", checked, "ties skipped", ties, "cut arcs", ones)
+    assert checked >= 300 and ones >= 100
     g.close()
-    N = n3 ** 3
+
+
+def test_config4_finite_differences(gpu):
+    """+-1 integer finite differences on 10^4 non-tie arcs of crops of the
+    config-4 volume's own capacities: 22 crops of 32^3 nodes centred on arcs
+    of the full solution's cut (arcs leaving a crop are dropped, so each crop
+    is a graph of its own with its own min cut).  Each E(w +- 1) is an exact
+    re-solve of the crop, 1024 perturbed crops solved per batched launch.
+    (The full volume is checked on fewer arcs above: 2 x 10^4 re-solves of
+    1.3e8 nodes would take minutes.)"""
+    gd = goldens().get("c4")
+    if gd is None:
+        pytest.skip("no golden for c4")
+    g, cs, ct, caps = _c4_graph(gpu, gd)
+    nx, ny, nz = gd["shape"]
+    g.solve(max_local_iters=4)
+    lab = g.labels()
+    g.close()
     rng = np.random.default_rng(0)
-    # half the sample on cut arcs (gradient 1), half anywhere
-    idx = np.arange(N).reshape(n3, n3, n3)
-    cand = []
-    for d, (dx, dy, dz) in enumerate(gpu.DIRS_3D):
-        src = idx[max(0, -dz):n3 - max(0, dz), max(0, -dy):n3 - max(0, dy), max(0, -dx):n3 - max(0, dx)].ravel()
-        ok = caps[d][src] >= 1
-        cand.append(np.stack([np.full(ok.sum(), d), src[ok]], 1))
-    cand = np.concatenate(cand)
-    cut = cand[grad[cand[:, 0], cand[:, 1]] == 1]
-    pick = np.concatenate([cut[rng.integers(0, len(cut), 6500)],
-                           cand[rng.integers(0, len(cand), 6500)]])
-    B = 512
+    cut = _cut_arcs(lab, caps, (nx, ny, nz), gpu.DIRS_3D)
+    n3 = 32
+    vol = lambda a: a.reshape(nz, ny, nx)
+    crops = []
+    for c in rng.choice(len(cut), 22, replace=False):
+        v = int(cut[c, 1])
+        z, y, x = v // (nx * ny), (v // nx) % ny, v % nx
+        z0, y0, x0 = (min(max(0, q - n3 // 2), n - n3) for q, n in ((z, nz), (y, ny), (x, nx)))
+        sl = (slice(z0, z0 + n3), slice(y0, y0 + n3), slice(x0, x0 + n3))
+        cc = np.stack([vol(caps[d])[sl] for d in range(6)]).copy()
+        # arcs leaving the crop are not part of its graph
+        cc[0, :, :, -1] = 0; cc[1, :, :, 0] = 0
+        cc[2, :, -1, :] = 0; cc[3, :, 0, :] = 0
+        cc[4, -1] = 0; cc[5, 0] = 0
+        crops.append((vol(cs)[sl].ravel().copy(), vol(ct)[sl].ravel().copy(),
+                      cc.reshape(6, -1)))
+    del cs, ct, caps, lab, cut
+    shape = (n3, n3, n3, 6)
+    N = n3 ** 3
     checked = ties = 0
-    for b0 in range(0, len(pick), B):
-        chunk = pick[b0:b0 + B]
-        G = gpu.GridGraph([(n3, n3, n3, 6)] * (2 * len(chunk)))
-        for i, (d, v) in enumerate(chunk):
+    B = 512
+    for ci, (ccs, cct, ccaps) in enumerate(crops):
+        h = gpu.GridGraph([shape])
+        h.set_terminal_caps(ccs, cct)
+        for d in range(6):
+            h.set_neighbor_caps(d, ccaps[d])
+        h.solve()
+        E0 = h.flow()
+        _, _, grad = h.backward()
+        h.close()
+        allarc = np.stack([rng.integers(0, 6, 20000), rng.integers(0, N, 20000)], 1)
+        allarc = allarc[ccaps[allarc[:, 0], allarc[:, 1]] >= 1]
+        ccut = allarc[grad[allarc[:, 0], allarc[:, 1]] == 1]
+        pick = np.concatenate([ccut[:B // 2], allarc[:B - min(len(ccut), B // 2)]])[:B]
+        G = gpu.GridGraph([shape] * (2 * len(pick)))
+        for i, (d, v) in enumerate(pick):
             for s, delta in enumerate((1, -1)):
                 inst = 2 * i + s
-                G.set_terminal_caps(cs, ct, inst=inst)
+                G.set_terminal_caps(ccs, cct, inst=inst)
                 for dd in range(6):
-                    p = caps[dd]
+                    p = ccaps[dd]
                     if dd == d:
                         p = p.copy()
                         p[v] += delta
@@ -175,14 +275,13 @@ def test_config4_finite_differences(gpu):
         G.solve()
         fl = G.flows()
         G.close()
-        for i, (d, v) in enumerate(chunk):
+        for i, (d, v) in enumerate(pick):
             up, dn = fl[2 * i] - E0, E0 - fl[2 * i + 1]
             if up != dn:
                 ties += 1
                 continue
-            assert grad[d][v] == up, (d, v, up)
+            assert grad[d][v] == up, (ci, d, v, up)
             checked += 1
-        if checked >= 10000:
-            break
+        say("crop", ci, "checked so far", checked, "ties", ties)
     say("finite differences checked", checked, "ties skipped", ties)
     assert checked >= 10000
