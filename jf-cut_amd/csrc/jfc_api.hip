@@ -122,6 +122,8 @@ struct jfc_graph {
   long long Nuser = 0;  // nodes of the instances as callers number them
   void *stage = nullptr;  // device staging of caller arrays (rows of nx)
   size_t stage_bytes = 0;
+  void *model_scratch = nullptr;  // data-term scratch (histograms, centres, sums)
+  size_t model_scratch_bytes = 0;
   std::vector<InstDesc> inst;
   std::vector<int> caps_set;  // per instance: bit 0 terminals, bit 1 + d neighbour planes
   std::vector<int> caps_failed;  // per instance: the same bits, for a setter that failed
@@ -237,6 +239,9 @@ struct jfc_graph {
     if (stage) (void)hipFree(stage);
     stage = nullptr;
     stage_bytes = 0;
+    if (model_scratch) (void)hipFree(model_scratch);
+    model_scratch = nullptr;
+    model_scratch_bytes = 0;
     if (priv_event) (void)hipEventDestroy(priv_event);
     if (priv_stream) (void)hipStreamDestroy(priv_stream);
     priv_event = nullptr;
@@ -386,6 +391,43 @@ void graph_mark_inputs(jfc_graph *g, int inst) {
   g->caps_failed[inst] = 0;
   g->mask_dirty = true;
   g->solved = false;
+}
+
+// A data-term call that failed after writing some of the instance's planes.
+void graph_inputs_failed(jfc_graph *g, int inst) {
+  g->caps_set[inst] = 0;
+  g->caps_failed[inst] = ~0;
+  g->mask_dirty = true;
+  g->solved = false;
+}
+
+hipStream_t graph_stream(jfc_graph *g) { return g->ustream; }
+
+int graph_stage(jfc_graph *g, size_t bytes, void **p);
+
+// The graph's scratch block for the data-term paths, allocated on first use
+// and grown when a call needs more.
+int graph_scratch(jfc_graph *g, size_t bytes, void **p) {
+  if (g->model_scratch_bytes < bytes) {
+    if (g->model_scratch) (void)hipFree(g->model_scratch);
+    g->model_scratch = nullptr;
+    g->model_scratch_bytes = 0;
+    if (hipMalloc(&g->model_scratch, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(JFC_ERR_OOM, "out of device memory: data-term scratch");
+    }
+    g->model_scratch_bytes = bytes;
+  }
+  *p = g->model_scratch;
+  return JFC_OK;
+}
+
+// The instance's slice of the excess plane (stored nodes), free between
+// solves: every solve rewrites it from the inputs before reading it.
+int graph_work_plane(jfc_graph *g, int inst, int **plane, long long *n) {
+  *plane = g->e + g->inst[inst].node_off;
+  *n = g->inst_stored(inst);
+  return JFC_OK;
 }
 }  // namespace jfc
 
@@ -577,6 +619,8 @@ static int stage_for(jfc_graph *g, size_t bytes, void **p) {
   *p = g->stage;
   return JFC_OK;
 }
+int jfc::graph_stage(jfc_graph *g, size_t bytes, void **p) { return stage_for(g, bytes, p); }
+
 static int sync_user(jfc_graph *g) {
   JFC_HIP(hipStreamSynchronize(g->ustream));
   return JFC_OK;
@@ -1049,20 +1093,21 @@ struct Solver {
     // the residual-arc masks restart from the inputs' with every plane
     // marked fresh: k_push reads fresh planes from c0 and writes them to cap
     const long long K = g->Kg, nodes = g->Nuser;
+    // one pass over the nodes folds the terminal arcs (read cs, ct, write e)
+    // and starts the masks: rebuilt from the input planes after an input
+    // change (read K planes, write mask0 and rmask), else copied (read
+    // mask0, write rmask)
+    const int masks = g->mask_dirty ? 2 : 1;
     if (g->mask_dirty) {
-      launch_build_mask(g->shape, T, P, g->mask0, st);
       g->mask_dirty = false;
       s.inputs_fresh = 1;
-      s.solve_bytes += nodes * (4 * K + 4);  // read K planes, write mask0
     }
-    s.solve_bytes += nodes * (8 + 12);  // mask0 -> rmask copy; fold: read cs, ct, write e
-    JFC_HIP(hipMemcpyAsync(g->rmask, g->mask0, (size_t)g->Ntot * sizeof(unsigned),
-                           hipMemcpyDeviceToDevice, st));
-    // Fast-Cut reads and writes the residual planes directly
-    if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
+    s.solve_bytes += nodes * (12 + (masks == 2 ? 4 * K + 8 : 8));
     if (int rc0 = g->ensure_pin()) return rc0;
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi,
-                o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, st);
+                o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, g->mask0, masks, st);
+    // Fast-Cut reads and writes the residual planes directly
+    if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     // the fold's sums are read with the results (no host wait here): an
     // instance without terminal capacity has no excess, so the solve passes
     // over it, and the error is reported at the end
@@ -1370,7 +1415,7 @@ extern "C" int jfc_debug_tile_flags(jfc_graph *g, int which, uint8_t *flags) {
   const int nI = (int)g->inst.size();
   launch_fill_i32((int *)g->d_red, 2LL * nI * RED_SLOTS, 0, 0);
   launch_fold(g->shape, g->tiles(), g->planes(), g->d_red, which == 2 ? g->d_flag : nullptr,
-              g->d_defi, false, 0);
+              g->d_defi, false, nullptr, 0, 0);
   unsigned char *src = which == 0 ? g->d_defi : g->d_flag;
   if (which == 1) {
     launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, 0);

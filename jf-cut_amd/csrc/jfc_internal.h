@@ -123,8 +123,11 @@ void launch_fill_multi(const FillSet &f, hipStream_t st);
 // zero_h: also write h = 0 over all nodes (the Fast-Cut kernels read every
 // height; the push kernel instead treats the heights of tiles it has not
 // visited in the solve as 0, see launch_push).
+// masks: 0 = rmask untouched; 1 = rmask = mask0; 2 = build mask0 from the
+// input planes and write it to rmask too (all in the same pass).
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, unsigned char *defi, bool zero_h, hipStream_t st);
+                 unsigned char *active, unsigned char *defi, bool zero_h, unsigned *mask0,
+                 int masks, hipStream_t st);
 void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
@@ -162,8 +165,6 @@ constexpr int kEntryMat = 1 << 30;
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const Queue &q, int *sweep_cnt,
                     long long *total, hipStream_t st);
-void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
-                       hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in
 // this solve (cleared when a solve starts).  defi: a visit rewrites its
 // tile's deficit flag (pushes never create deficits, so unvisited tiles'

@@ -3,29 +3,44 @@
 // src/QVisualizer/Graph Cut/QGCWidget.cpp:2560-2606).
 //
 // Reference counterparts (src/QVisualizer/cl/GraphCut/):
-//   k_seed_list   graphcut_count_cut / graphcut_scatter_cut  graphcut_count_block.cl:67-104
-//   k_key_hist    graphcut_cut_distribution                  graphcut_histogram.cl:41-60
+//   k_seed_scan   graphcut_count_cut / graphcut_scatter_cut  graphcut_count_block.cl:67-104
+//                 + graphcut_cut_distribution                graphcut_histogram.cl:41-60
+//   k_km_init     the host pick of initial centres           QGCWidget.cpp:2001-2039
 //   k_km_assign   graphcut_kmeans                            graphcut_kmeans.cl:21-89
+//   k_km_update   the host centre update                     QGCWidget.cpp:2073-2102
 //   k_unary       graphcut_compute_excessflow_lazy           graphcut_compute_excessflow.cl:47-77
 //   k_pairwise    graphcut_compute_capacity_bk               graphcut_compute_capacity.cl:21-69
 //
 // The reference runs k-means in float32 with host-side float division; here
 // every quantity is an integer so CPU and GPU agree bit for bit: centroids
 // are fixed point (value * 2^8) rounded from exact int64 sums, distances are
-// exact int64, unary rounding is integer, and the pairwise exp() is replaced
-// by a table of integer thresholds on the squared difference, computed once
-// on the host.
+// exact integers, unary rounding is integer, and the pairwise exp() becomes a
+// table of integer thresholds on the squared difference, computed once per
+// call on the host.
+//
+// The whole model runs on the caller's stream with three host waits (the
+// seed counts, the pair sums for the thresholds, the end); the k-means
+// iterations run on the device (each update raises a done flag that turns
+// the later launches into no-ops).  The scratch is one small block the graph
+// allocates on its first model call, plus a seed list.
 #include "jfcut.h"
 #include "jfc_device.h"
 
 #include <cmath>
 #include <cstring>
 #include <string>
-#include <vector>
 
 namespace jfc {
 int model_fail(int code, const std::string &msg);
-}
+int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int **c0,
+                   long long *stride);
+void graph_mark_inputs(jfc_graph *g, int inst);
+void graph_inputs_failed(jfc_graph *g, int inst);
+hipStream_t graph_stream(jfc_graph *g);
+int graph_stage(jfc_graph *g, size_t bytes, void **p);
+int graph_scratch(jfc_graph *g, size_t bytes, void **p);
+int graph_work_plane(jfc_graph *g, int inst, int **plane, long long *n);
+}  // namespace jfc
 
 using namespace jfc;
 
@@ -33,70 +48,198 @@ namespace {
 
 constexpr int KM_FRAC = 8;  // fixed-point bits of centroids
 constexpr int KMAX_CLUSTERS = 16;
-constexpr int TMAX = 256;   // max thresholds per table (lambda < 255.5)
+constexpr int TMAX = 256;        // max thresholds per table (lambda < 255.5)
+constexpr int NKEYS_MAX = 4096;  // 255 * 3 + 1 keys (RGB) or 4096 (uint16 >> 4)
 
-__device__ __forceinline__ int feat(const void *img, int bpc, long long i) {
-  return bpc == 1 ? (int)((const unsigned char *)img)[i]
-                  : (int)((const unsigned short *)img)[i];
+// Scratch of one model call: a small block (histograms, centroids, sums,
+// thresholds, control words) and a seed list.
+struct Scratch {
+  long long *kcnt;  // [2][NKEYS_MAX] seeded pixels per key
+  long long *ksum;  // [2][NKEYS_MAX] feature sums per key and channel (key * C + c)
+  long long *cent;  // [2][KMAX_CLUSTERS * 3] centroids, fixed point
+  long long *acc;   // [2][KMAX_CLUSTERS * 4] cluster sums and counts of an iteration
+  long long *sums;  // [4] S0, P0, S1, P1: squared differences and edges per group
+  unsigned *T;      // [3][TMAX] pairwise thresholds
+  int *ctl;         // [0,1] seed counts, [2,3] done flags, [4,5] iterations
+  int *list;        // seeded node indices: foreground from the front, background
+                    // from the back (list[cap - 1 - i])
+  long long cap;    // list entries (both sides together)
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kKcntBytes = align256(2 * NKEYS_MAX * 8);
+constexpr size_t kKsumBytes = align256(2 * NKEYS_MAX * 8);
+constexpr size_t kCentBytes = align256(2 * KMAX_CLUSTERS * 3 * 8);
+constexpr size_t kAccBytes = align256(2 * KMAX_CLUSTERS * 4 * 8);
+constexpr size_t kSumsBytes = align256(4 * 8);
+constexpr size_t kTBytes = align256(3 * TMAX * 4);
+constexpr size_t kCtlBytes = align256(8 * 4);
+constexpr size_t kSmallBytes =
+    kKcntBytes + kKsumBytes + kCentBytes + kAccBytes + kSumsBytes + kTBytes + kCtlBytes;
+// The part that must start at zero in every call: histograms, accumulators,
+// sums and control words (cent and T are written before they are read).
+constexpr size_t kClearInts = (kKcntBytes + kKsumBytes) / 4;
+
+// The small part of the scratch from `base`; the seed list is given apart.
+Scratch carve(void *base, int *list, long long cap) {
+  unsigned char *p = (unsigned char *)base;
+  Scratch s;
+  s.kcnt = (long long *)p; p += kKcntBytes;
+  s.ksum = (long long *)p; p += kKsumBytes;
+  s.cent = (long long *)p; p += kCentBytes;
+  s.acc = (long long *)p; p += kAccBytes;
+  s.sums = (long long *)p; p += kSumsBytes;
+  s.T = (unsigned *)p; p += kTBytes;
+  s.ctl = (int *)p;
+  s.list = list;
+  s.cap = cap;
+  return s;
 }
 
-__global__ void k_seed_list(const unsigned char *seeds, long long n, int side,
-                            int *list, int *count) {
+// Seeded pixels per key: the plain sum of the channels (8-bit) or the top 12
+// bits (16-bit), the levels graphcut_cut_distribution bins.
+template <int BPC>
+__device__ __forceinline__ int pixel(const void *img, long long i) {
+  return BPC == 1 ? (int)((const unsigned char *)img)[i] : (int)((const unsigned short *)img)[i];
+}
+
+// Zero the histograms and, in one launch, the accumulators, sums and control
+// words (the scratch keeps the previous call's contents).
+__global__ void k_model_clear(Scratch s, int iters) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = tid; i < (int)kClearInts; i += stride) ((int *)s.kcnt)[i] = 0;
+  for (int i = tid; i < 2 * KMAX_CLUSTERS * 4; i += stride) s.acc[i] = 0;
+  if (tid < 4) s.sums[tid] = 0;
+  if (tid < 4) s.ctl[tid] = 0;             // seed counts, done flags
+  if (tid == 4 || tid == 5) s.ctl[tid] = iters;  // iterations unless done earlier
+}
+
+// One pass over the seed map: seeded node lists per side (an entry past the
+// list's end is dropped; the count keeps counting so that the host sees the
+// overflow) and the key histograms of the seeded pixels.
+template <int BPC>
+__global__ __launch_bounds__(256) void k_seed_scan(const void *img, int C,
+                                                   const unsigned char *seeds, long long n,
+                                                   Scratch s) {
+  const int lane = lane_id();
   for (long long base = (long long)blockIdx.x * blockDim.x; base < n;
        base += (long long)gridDim.x * blockDim.x) {
     const long long i = base + threadIdx.x;
-    const bool p = i < n && seeds[i] == side;
-    wave_append(p, (int)i, list, count, nullptr, nullptr);
-  }
-}
-
-__global__ void k_key_hist(const void *img, int bpc, int C, const int *list,
-                           const int *count, long long *kcnt, long long *ksum) {
-  const int n = *count;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const long long v = list[i];
-    int key = 0;
-    int x[3];
-    for (int c = 0; c < C; ++c) {
-      x[c] = feat(img, bpc, v * C + c);
-      key += x[c];
+    const int sd = i < n ? seeds[i] : 0;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const bool p = sd == side + 1;
+      const unsigned long long m = __ballot(p);
+      if (!m) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      int b = 0;
+      if (lane == leader) b = atomicAdd(&s.ctl[side], __popcll(m));
+      b = __shfl(b, leader);
+      if (p) {
+        const long long at = b + __popcll(m & ((1ull << lane) - 1ull));
+        if (at < s.cap) s.list[side ? s.cap - 1 - at : at] = (int)i;
+        int x[3] = {0, 0, 0}, key = 0;
+        for (int c = 0; c < C; ++c) {
+          x[c] = pixel<BPC>(img, i * C + c);
+          key += x[c];
+        }
+        if (BPC == 2) key = x[0] >> 4;
+        atomicAdd((unsigned long long *)&s.kcnt[side * NKEYS_MAX + key], 1ull);
+        for (int c = 0; c < C; ++c)
+          atomicAdd((unsigned long long *)&s.ksum[side * NKEYS_MAX + key * C + c],
+                    (unsigned long long)x[c]);
+      }
     }
-    if (bpc == 2) key = x[0] >> 4;
-    atomicAdd((unsigned long long *)&kcnt[key], 1ull);
-    for (int c = 0; c < C; ++c)
-      atomicAdd((unsigned long long *)&ksum[(long long)key * C + c], (unsigned long long)x[c]);
   }
 }
 
+// Initial centroids per side (one workgroup of 1024 lanes, blockIdx.x =
+// side): the distinct seeded keys in ascending order, the j-th centre taken
+// from key number j * (m - 1) / (K - 1), rounded to fixed point
+// (QGCWidget.cpp:2001-2039).
+__global__ __launch_bounds__(1024) void k_km_init(Scratch s, int C, int nkeys, int Kf, int Kb) {
+  __shared__ int s_cnt[1024];
+  __shared__ long long s_pick[KMAX_CLUSTERS];
+  const int side = blockIdx.x, K = side ? Kb : Kf, tid = threadIdx.x;
+  const long long *kc = s.kcnt + side * NKEYS_MAX;
+  const long long *ks = s.ksum + side * NKEYS_MAX;
+  long long *cent = s.cent + side * KMAX_CLUSTERS * 3;
+  int mine = 0;
+  for (int q = 0; q < 4; ++q) {
+    const int k = tid * 4 + q;
+    if (k < nkeys && kc[k]) ++mine;
+  }
+  s_cnt[tid] = mine;
+  __syncthreads();
+  // inclusive scan over the 1024 counts
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = tid >= o ? s_cnt[tid - o] : 0;
+    __syncthreads();
+    s_cnt[tid] += v;
+    __syncthreads();
+  }
+  const int m = s_cnt[1023];
+  if (tid < K) s_pick[tid] = K > 1 ? (long long)tid * (m - 1) / (K - 1) : 0;
+  if (tid < K * C) cent[tid] = 0;
+  __syncthreads();
+  if (m == 0) return;  // no seeds: reported by the host
+  int r = s_cnt[tid] - mine;  // rank of this lane's first non-empty key
+  for (int q = 0; q < 4; ++q) {
+    const int k = tid * 4 + q;
+    if (k >= nkeys || !kc[k]) continue;
+    for (int j = 0; j < K; ++j)
+      if (s_pick[j] == r) {
+        const long long nn = kc[k];
+        for (int c = 0; c < C; ++c)
+          cent[j * C + c] = (ks[k * C + c] * (1 << KM_FRAC) + nn / 2) / nn;
+      }
+    ++r;
+  }
+}
+
+// Squared distance between a feature and a fixed-point centroid.  8-bit
+// channels: |x * 256 - c| < 2^16, so each square fits 32 bits unsigned;
+// 16-bit: 64-bit products.
+template <int BPC>
 __device__ __forceinline__ long long km_d2(const int *x, const long long *c, int C) {
-  long long d = 0;
-  for (int i = 0; i < C; ++i) {
-    const long long t = (long long)x[i] * (1 << KM_FRAC) - c[i];
-    d += t * t;
+  if (BPC == 1) {
+    unsigned long long d = 0;
+    for (int i = 0; i < C; ++i) {
+      const int t = x[i] * (1 << KM_FRAC) - (int)c[i];
+      d += (unsigned)(t * t);
+    }
+    return (long long)d;
+  } else {
+    const long long t = (long long)x[0] * (1 << KM_FRAC) - c[0];
+    return t * t;
   }
-  return d;
 }
 
-// Assignment to the nearest centroid (lowest index on ties) and per-cluster
-// sums, reduced in LDS before one global atomic per cluster entry.
-__global__ __launch_bounds__(256) void k_km_assign(const void *img, int bpc, int C,
-                                                   const int *list, const int *count,
-                                                   const long long *cent, int K,
-                                                   long long *acc) {
+// Lloyd assignment (graphcut_kmeans.cl:21-89): nearest centroid, lowest
+// index on ties; cluster sums and counts reduced in LDS, then one global
+// atomic per entry and workgroup.  blockIdx.y = side; a side whose
+// iteration has converged (done flag) returns at once.
+template <int BPC>
+__global__ __launch_bounds__(256) void k_km_assign(const void *img, int C, Scratch s, int Kf,
+                                                   int Kb) {
+  const int side = blockIdx.y, K = side ? Kb : Kf;
+  if (s.ctl[2 + side]) return;
   __shared__ long long s_c[KMAX_CLUSTERS * 3];
   __shared__ long long s_acc[KMAX_CLUSTERS * 4];
-  for (int i = threadIdx.x; i < K * C; i += blockDim.x) s_c[i] = cent[i];
+  for (int i = threadIdx.x; i < K * C; i += blockDim.x) s_c[i] = s.cent[side * KMAX_CLUSTERS * 3 + i];
   for (int i = threadIdx.x; i < K * (C + 1); i += blockDim.x) s_acc[i] = 0;
   __syncthreads();
-  const int n = *count;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const long long v = list[i];
-    int x[3];
-    for (int c = 0; c < C; ++c) x[c] = feat(img, bpc, v * C + c);
+  const long long n = min((long long)s.ctl[side], s.cap);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long v = s.list[side ? s.cap - 1 - i : i];
+    int x[3] = {0, 0, 0};
+    for (int c = 0; c < C; ++c) x[c] = pixel<BPC>(img, v * C + c);
     int best = 0;
-    long long bd = km_d2(x, s_c, C);
+    long long bd = km_d2<BPC>(x, s_c, C);
     for (int k = 1; k < K; ++k) {
-      const long long d = km_d2(x, s_c + k * C, C);
+      const long long d = km_d2<BPC>(x, s_c + k * C, C);
       if (d < bd) { bd = d; best = k; }
     }
     for (int c = 0; c < C; ++c)
@@ -104,80 +247,165 @@ __global__ __launch_bounds__(256) void k_km_assign(const void *img, int bpc, int
     atomicAdd((unsigned long long *)&s_acc[best * (C + 1) + C], 1ull);
   }
   __syncthreads();
+  long long *acc = s.acc + side * KMAX_CLUSTERS * 4;
   for (int i = threadIdx.x; i < K * (C + 1); i += blockDim.x)
     if (s_acc[i]) atomicAdd((unsigned long long *)&acc[i], (unsigned long long)s_acc[i]);
 }
 
-// cs = round(scale * d2_bg / den), ct = round(scale * d2_fg / den); seeds get
-// (2^30, 0) / (0, 2^30).
-// v: the image's (unpadded) node index; the planes store row r at r * px.
-__global__ void k_unary(const void *img, int bpc, int C, long long n, int nx, int px,
-                        const unsigned char *seeds, const long long *cfg, int Kf,
-                        const long long *cbg, int Kb, long long den, long long scale,
-                        int *cs, int *ct) {
-  __shared__ long long s_f[KMAX_CLUSTERS * 3], s_b[KMAX_CLUSTERS * 3];
-  for (int i = threadIdx.x; i < Kf * C; i += blockDim.x) s_f[i] = cfg[i];
-  for (int i = threadIdx.x; i < Kb * C; i += blockDim.x) s_b[i] = cbg[i];
+// Centre update of iteration `it` (QGCWidget.cpp:2088-2101): the rounded mean
+// of each non-empty cluster; a side where no centre moved is done after
+// `it` iterations.  The accumulators are cleared for the next iteration.
+// One workgroup of 64 lanes per side (blockIdx.x).
+__global__ __launch_bounds__(64) void k_km_update(Scratch s, int C, int Kf, int Kb, int it) {
+  const int side = blockIdx.x, K = side ? Kb : Kf;
+  if (s.ctl[2 + side]) return;
+  long long *acc = s.acc + side * KMAX_CLUSTERS * 4;
+  long long *cent = s.cent + side * KMAX_CLUSTERS * 3;
+  const int j = threadIdx.x;  // (cluster, channel) = (j / C, j % C)
+  int changed = 0;
+  long long nc = 0;
+  if (j < K * C) {
+    const int k = j / C, c = j % C;
+    const long long nk = acc[k * (C + 1) + C];
+    nc = cent[j];
+    if (nk) {
+      const long long v = (acc[k * (C + 1) + c] * (1 << KM_FRAC) + nk / 2) / nk;
+      changed = v != nc;
+      nc = v;
+    }
+  }
+  const int any = __ballot(changed) != 0;
   __syncthreads();
-  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n;
-       v += (long long)gridDim.x * blockDim.x) {
-    const long long sv = (v / nx) * px + v % nx;
-    const int sd = seeds ? seeds[v] : 0;
-    if (sd == 1) { cs[sv] = 1 << 30; ct[sv] = 0; continue; }
-    if (sd == 2) { cs[sv] = 0; ct[sv] = 1 << 30; continue; }
-    int x[3];
-    for (int c = 0; c < C; ++c) x[c] = feat(img, bpc, v * C + c);
-    long long df = km_d2(x, s_f, C), db = km_d2(x, s_b, C);
-    for (int k = 1; k < Kf; ++k) df = min(df, km_d2(x, s_f + k * C, C));
-    for (int k = 1; k < Kb; ++k) db = min(db, km_d2(x, s_b + k * C, C));
-    cs[sv] = (int)((scale * db + den / 2) / den);
-    ct[sv] = (int)((scale * df + den / 2) / den);
+  if (j < K * C) cent[j] = nc;
+  for (int i = j; i < K * (C + 1); i += 64) acc[i] = 0;
+  if (j == 0 && !any) {
+    s.ctl[2 + side] = 1;
+    s.ctl[4 + side] = it;
   }
 }
 
-template <bool D3>
-__device__ __forceinline__ bool nb_at(const InstDesc &I, int x, int y, int z, int d,
-                                      long long &u) {
-  int dx, dy, dz;
-  dir_of<D3>(d, dx, dy, dz);
-  const int X = x + dx, Y = y + dy, Z = z + dz;
-  if (X < 0 || X >= I.nx || Y < 0 || Y >= I.ny || Z < 0 || Z >= I.nz) return false;
-  u = X + (long long)I.nx * (Y + (long long)I.ny * Z);
-  return true;
+// round(q) of a non-negative integer quotient x / den, exactly: the double
+// quotient is within one of the floor (x / den < 2^31 here), then one integer
+// correction step.
+__device__ __forceinline__ long long div_floor(long long x, long long den) {
+  long long q = (long long)((double)x / (double)den);
+  if (q * den > x) --q;
+  else if ((q + 1) * den <= x) ++q;
+  return q;
 }
 
-__device__ __forceinline__ long long sqdiff(const void *img, int bpc, int C, long long a,
-                                            long long b) {
-  long long D = 0;
+// cs = round(scale * d2_bg / den), ct = round(scale * d2_fg / den); seeds get
+// (2^30, 0) / (0, 2^30).  Rows of the image (unpadded, nx nodes) go to rows
+// of the planes (pitch px); lanes walk a row, workgroups walk rows.
+template <int BPC>
+__global__ __launch_bounds__(256) void k_unary(const void *img, int C, const unsigned char *seeds,
+                                               int nx, long long rows, int px, Scratch s,
+                                               int Kf, int Kb, long long den, long long scale,
+                                               int *cs, int *ct) {
+  __shared__ long long s_f[KMAX_CLUSTERS * 3], s_b[KMAX_CLUSTERS * 3];
+  for (int i = threadIdx.x; i < Kf * C; i += blockDim.x) s_f[i] = s.cent[i];
+  for (int i = threadIdx.x; i < Kb * C; i += blockDim.x) s_b[i] = s.cent[KMAX_CLUSTERS * 3 + i];
+  __syncthreads();
+  const long long half = den / 2;
+  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+    const long long v0 = r * nx, o0 = r * px;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+      const long long v = v0 + x;
+      const int sd = seeds[v];
+      int a, b;
+      if (sd == 1) { a = 1 << 30; b = 0; }
+      else if (sd == 2) { a = 0; b = 1 << 30; }
+      else {
+        int f[3] = {0, 0, 0};
+        for (int c = 0; c < C; ++c) f[c] = pixel<BPC>(img, v * C + c);
+        long long df = km_d2<BPC>(f, s_f, C), db = km_d2<BPC>(f, s_b, C);
+        for (int k = 1; k < Kf; ++k) df = min(df, km_d2<BPC>(f, s_f + k * C, C));
+        for (int k = 1; k < Kb; ++k) db = min(db, km_d2<BPC>(f, s_b + k * C, C));
+        a = (int)div_floor(scale * db + half, den);
+        b = (int)div_floor(scale * df + half, den);
+      }
+      cs[o0 + x] = a;
+      ct[o0 + x] = b;
+    }
+  }
+}
+
+// Squared feature difference of two nodes.
+template <int BPC>
+__device__ __forceinline__ unsigned long long sqdiff(const void *img, int C, long long a,
+                                                     long long b) {
+  unsigned long long D = 0;
   for (int c = 0; c < C; ++c) {
-    const long long t = feat(img, bpc, a * C + c) - feat(img, bpc, b * C + c);
-    D += t * t;
+    const long long t = pixel<BPC>(img, a * C + c) - pixel<BPC>(img, b * C + c);
+    D += (unsigned long long)(t * t);
   }
   return D;
 }
 
-// Sum of squared neighbour differences per edge group (0: x/y and
-// diagonals, 1: z when it forms its own group), one arc per undirected edge.
-template <bool D3>
-__global__ __launch_bounds__(256) void k_pair_sums(InstDesc I, const void *img, int bpc,
-                                                   int C, int zsep, long long *sums) {
+// The undirected edges a node owns: the forward directions (even d) whose
+// neighbour lies inside the grid; each edge is owned by exactly one node.
+// 2D: +x, +y and, 8-connected, (+x,+y), (-x,+y); 3D: +x, +y, +z.
+template <int CONN>
+struct Fwd {
+  static constexpr int N = CONN == 8 ? 4 : (CONN == 6 ? 3 : 2);
+  static __device__ __forceinline__ int dir(int j) { return 2 * j; }
+  static __device__ __forceinline__ bool inside(int j, int x, int y, int z, int nx, int ny,
+                                                int nz) {
+    switch (2 * j) {
+      case 0: return x + 1 < nx;
+      case 2: return y + 1 < ny;
+      case 4: return CONN == 6 ? z + 1 < nz : (x + 1 < nx && y + 1 < ny);
+      default: return x >= 1 && y + 1 < ny;  // 6: (-x, +y)
+    }
+  }
+  // neighbour offset in the unpadded numbering (row r = y + ny z)
+  static __device__ __forceinline__ long long off(int j, int nx, long long plane) {
+    switch (2 * j) {
+      case 0: return 1;
+      case 2: return nx;
+      case 4: return CONN == 6 ? plane : nx + 1;
+      default: return nx - 1;
+    }
+  }
+  // the same in the stored numbering (rows of pitch px)
+  static __device__ __forceinline__ long long soff(int j, int px, long long splane) {
+    switch (2 * j) {
+      case 0: return 1;
+      case 2: return px;
+      case 4: return CONN == 6 ? splane : px + 1;
+      default: return px - 1;
+    }
+  }
+  // threshold table of the edge: 0 = x/y, 1 = 2D diagonal, 2 = z
+  static __device__ __forceinline__ int table(int j) {
+    return 2 * j >= 4 ? (CONN == 6 ? 2 : 1) : 0;
+  }
+};
+
+// Sum of squared neighbour differences per edge group (0: x/y and the 2D
+// diagonals, and z unless it forms its own group; 1: z when separate), one
+// term per undirected edge.
+template <int CONN, int BPC>
+__global__ __launch_bounds__(256) void k_pair_sums(const void *img, int C, int nx, int ny,
+                                                   int nz, int zsep, long long *sums) {
   __shared__ long long buf[4][4];
-  long long s0 = 0, p0 = 0, s1 = 0, p1 = 0;
-  const long long rows = (long long)I.ny * I.nz;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int y = (int)(row % I.ny), z = (int)(row / I.ny);
-    for (int x = threadIdx.x; x < I.nx; x += blockDim.x) {
-      const long long v = x + (long long)I.nx * row;
-      for (int d = 0; d < I.K; d += 2) {
-        long long u;
-        if (!nb_at<D3>(I, x, y, z, d, u)) continue;
-        const long long D = sqdiff(img, bpc, C, v, u);
-        if (D3 && d == 4 && zsep) { s1 += D; p1 += 1; }
-        else { s0 += D; p0 += 1; }
+  unsigned long long s0 = 0, s1 = 0;
+  long long p0 = 0, p1 = 0;
+  const long long rows = (long long)ny * nz, plane = (long long)nx * ny;
+  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+      const long long v = r * nx + x;
+#pragma unroll
+      for (int j = 0; j < Fwd<CONN>::N; ++j) {
+        if (!Fwd<CONN>::inside(j, x, y, z, nx, ny, nz)) continue;
+        const unsigned long long D = sqdiff<BPC>(img, C, v, v + Fwd<CONN>::off(j, nx, plane));
+        if (CONN == 6 && j == 2 && zsep) { s1 += D; ++p1; }
+        else { s0 += D; ++p0; }
       }
     }
   }
-  long long vals[4] = {s0, p0, s1, p1};
+  long long vals[4] = {(long long)s0, p0, (long long)s1, p1};
   const int w = threadIdx.x >> 6;
   for (int j = 0; j < 4; ++j) {
     long long x = vals[j];
@@ -192,84 +420,85 @@ __global__ __launch_bounds__(256) void k_pair_sums(InstDesc I, const void *img, 
   }
 }
 
-// w(D) = #{k : D <= T[k]} with tables 0: x/y, 1: 2D diagonals, 2: z.  Every
-// table is non-increasing (thresholds() below), so the count is the length
-// of the prefix with D <= T[k], found by binary search.
-template <bool D3>
-__global__ __launch_bounds__(256) void k_pairwise(InstDesc I, const void *img, int bpc,
-                                                  int C, const long long *T,
-                                                  const int *nT, int *c0,
-                                                  long long stride) {
-  __shared__ long long s_T[3 * TMAX];
-  __shared__ int s_n[3];
+// Pairwise weights w(D) = #{k : D <= T[k]} for each undirected edge, written
+// to both of its arcs (the weight is symmetric): plane d at the owner and
+// plane d^1 at the neighbour; arcs that leave the grid get 0 from the node
+// they start at.  Every table is non-increasing, so w is the length of the
+// prefix with D <= T[k]: a guess from the continuous form lambda exp(-D /
+// (2 s2)) + 1/2 is corrected against the exact integer table.
+struct PairParams {
+  int n[3];          // thresholds per table
+  float lam[3];      // lambda per table (the guess)
+  float inv2s2[3];   // 1 / (2 s2) per table (the guess)
+};
+
+template <int CONN, int BPC>
+__global__ __launch_bounds__(256) void k_pairwise(const void *img, int C, int nx, int ny,
+                                                  int nz, int px, const unsigned *T,
+                                                  PairParams pp, int *c0, long long stride) {
+  __shared__ unsigned s_T[3 * TMAX];
   for (int i = threadIdx.x; i < 3 * TMAX; i += blockDim.x) s_T[i] = T[i];
-  if (threadIdx.x < 3) s_n[threadIdx.x] = nT[threadIdx.x];
   __syncthreads();
-  const long long rows = (long long)I.ny * I.nz;
-  for (long long row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int y = (int)(row % I.ny), z = (int)(row / I.ny);
-    for (int x = threadIdx.x; x < I.nx; x += blockDim.x) {
-      const long long v = x + (long long)I.nx * row;
-      for (int d = 0; d < I.K; ++d) {
-        long long u;
+  const long long rows = (long long)ny * nz, plane = (long long)nx * ny,
+                  splane = (long long)px * ny;
+  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+      const long long v = r * nx + x, sv = r * px + x;
+#pragma unroll
+      for (int j = 0; j < Fwd<CONN>::N; ++j) {
+        const int d = Fwd<CONN>::dir(j);
         int w = 0;
-        if (nb_at<D3>(I, x, y, z, d, u)) {
-          const long long D = sqdiff(img, bpc, C, v, u);
-          const int tb = D3 ? (d >= 4 ? 2 : 0) : (d >= 4 ? 1 : 0);
-          const long long *t = s_T + tb * TMAX;
-          int hi = s_n[tb];
-          while (w < hi) {
-            const int mid = (w + hi) >> 1;
-            if (D <= t[mid]) w = mid + 1;
-            else hi = mid;
-          }
+        if (Fwd<CONN>::inside(j, x, y, z, nx, ny, nz)) {
+          const unsigned long long D = sqdiff<BPC>(img, C, v, v + Fwd<CONN>::off(j, nx, plane));
+          const int tb = Fwd<CONN>::table(j);
+          const int n = pp.n[tb];
+          const unsigned *t = s_T + tb * TMAX;
+          w = (int)(pp.lam[tb] * __expf(-(float)D * pp.inv2s2[tb]) + 0.5f);
+          w = max(0, min(w, n));
+          while (w < n && D <= t[w]) ++w;
+          while (w > 0 && D > t[w - 1]) --w;
+          c0[(d ^ 1) * stride + sv + Fwd<CONN>::soff(j, px, splane)] = w;
         }
-        c0[d * stride + I.node_off + x + (long long)I.px * row] = w;
+        c0[d * stride + sv] = w;
+      }
+      // reverse arcs of this node that leave the grid
+      if (x == 0) c0[1 * stride + sv] = 0;
+      if (y == 0) c0[3 * stride + sv] = 0;
+      if (CONN == 6 && z == 0) c0[5 * stride + sv] = 0;
+      if (CONN == 8) {
+        if (x == 0 || y == 0) c0[5 * stride + sv] = 0;
+        if (x + 1 == nx || y == 0) c0[7 * stride + sv] = 0;
       }
     }
   }
 }
 
-int thresholds(double lambda, long long S, long long P, long long *T) {
+int thresholds(double lambda, long long S, long long P, unsigned *T) {
   int kmax = (int)std::floor(lambda + 0.5);
   if (kmax > TMAX) kmax = TMAX;
   for (int k = 1; k <= kmax; ++k) {
-    if (S == 0 || P == 0) { T[k - 1] = INT64_MAX; continue; }
+    if (S == 0 || P == 0) { T[k - 1] = 0xffffffffu; continue; }
     const double v = (2.0 * (double)S / (double)P) * std::log(lambda / (k - 0.5));
-    T[k - 1] = (long long)std::floor(v);
+    // differences are below 2^32, so a threshold at or above 2^32 - 1 admits
+    // every difference, like the int64 value it stands for
+    const double f = std::floor(v);
+    T[k - 1] = f >= 4294967295.0 ? 0xffffffffu : (unsigned)f;
   }
   return kmax;
 }
 
-int grid1(long long n) {
+int grid1(long long n, int cap = 8192) {
   long long g = (n + 255) / 256;
-  return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+  return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
-
-}  // namespace
-
-// Implemented in jfc_api.hip: accessors of the opaque graph for this file.
-namespace jfc {
-int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int **c0,
-                   long long *stride);
-void graph_mark_inputs(jfc_graph *g, int inst);
-}
-
-#define MODEL_HIP(call)                                                         \
-  do {                                                                          \
-    hipError_t e_ = (call);                                                     \
-    if (e_ != hipSuccess) {                                                     \
-      rc = model_fail(JFC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-      goto done;                                                                \
-    }                                                                           \
-  } while (0)
 
 // Parameter checks that need no graph.  The unary of a node is
 // round(scale * d2 / den) with d2 <= C * (maxval * 2^KM_FRAC)^2 (a feature and
 // a centroid both in [0, maxval]) and den = range^2 * 2^(2 KM_FRAC): the
 // product scale * d2 must fit int64 and the unary must not exceed the
 // terminal-capacity limit, or the solve's int32 arithmetic would not be exact.
-static int check_model_params(int channels, int bpc, const jfc_model_params *prm) {
+int check_model_params(int channels, int bpc, const jfc_model_params *prm) {
   if (!prm) return model_fail(JFC_ERR_INVALID, "null argument");
   if (!((channels == 1 || channels == 3) && (bpc == 1 || (bpc == 2 && channels == 1))))
     return model_fail(JFC_ERR_INVALID, "image must be 1 or 3 channels of uint8, or 1 channel of uint16");
@@ -277,7 +506,7 @@ static int check_model_params(int channels, int bpc, const jfc_model_params *prm
     return model_fail(JFC_ERR_INVALID, "cluster count must be 1..16");
   if (prm->range <= 0 || prm->range > (1 << 20) || prm->unary_scale < 0 ||
       !(prm->lambda_xy >= 0) || !(prm->lambda_z >= 0) || prm->lambda_xy > 255 ||
-      prm->lambda_z > 255)
+      prm->lambda_z > 255 || prm->kmeans_iters < 0)
     return model_fail(JFC_ERR_INVALID, "invalid model parameters");
   const long double maxval = bpc == 1 ? 255.0L : 65535.0L;
   const long double d2max = (long double)channels * (maxval * (1 << KM_FRAC)) * (maxval * (1 << KM_FRAC));
@@ -290,144 +519,156 @@ static int check_model_params(int channels, int bpc, const jfc_model_params *prm
   return JFC_OK;
 }
 
+#define MODEL_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return model_fail(JFC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// The model of one instance from device-resident image and seeds, with the
+// given scratch, on stream st.  Writes the instance's cs, ct and c0 planes.
+template <int BPC>
+int run_model(const InstDesc &I, const void *img, int C, const unsigned char *seeds,
+              const jfc_model_params *prm, jfc_model_stats &ST, Scratch s, int *cs, int *ct,
+              int *c0, long long stride, hipStream_t st, bool *planes_written) {
+  *planes_written = false;
+  const long long n = (long long)I.nx * I.ny * I.nz;
+  const long long rows = (long long)I.ny * I.nz;
+  const int nkeys = BPC == 2 ? 4096 : 255 * C + 1;
+  const int iters = prm->kmeans_iters;
+  k_model_clear<<<64, 256, 0, st>>>(s, iters);
+  k_seed_scan<BPC><<<grid1(n, 16384), 256, 0, st>>>(img, C, seeds, n, s);
+  int ctl[6];
+  MODEL_HIP(hipMemcpyAsync(ctl, s.ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  MODEL_HIP(hipStreamSynchronize(st));
+  ST.seeds_fg = ctl[0];
+  ST.seeds_bg = ctl[1];
+  if (ctl[0] == 0 || ctl[1] == 0)
+    return model_fail(JFC_ERR_NO_SEEDS, ctl[0] == 0 ? "no foreground seeds" : "no background seeds");
+  if ((long long)ctl[0] + ctl[1] > s.cap)
+    return model_fail(JFC_ERR_INVALID, "more seeded nodes (" +
+                                           std::to_string((long long)ctl[0] + ctl[1]) +
+                                           ") than the seed list holds (" +
+                                           std::to_string(s.cap) + ")");
+  // k-means on the device: iteration `it` assigns and updates; a side whose
+  // centres stop moving is done and its later launches return at once
+  k_km_init<<<2, 1024, 0, st>>>(s, C, nkeys, prm->k_fg, prm->k_bg);
+  const long long most = std::max(ctl[0], ctl[1]);
+  const dim3 ag(std::min(grid1(most), 1024), 2);
+  for (int it = 1; it <= iters; ++it) {
+    k_km_assign<BPC><<<ag, 256, 0, st>>>(img, C, s, prm->k_fg, prm->k_bg);
+    k_km_update<<<2, 64, 0, st>>>(s, C, prm->k_fg, prm->k_bg, it);
+  }
+  // unary terms from the final centres, straight into the stored rows
+  const long long den = prm->range * prm->range * (1LL << (2 * KM_FRAC));
+  const int rgrid = (int)std::min<long long>(rows, 8192);
+  k_unary<BPC><<<rgrid, 256, 0, st>>>(img, C, seeds, I.nx, rows, I.px, s, prm->k_fg,
+                                      prm->k_bg, den, prm->unary_scale, cs + I.node_off,
+                                      ct + I.node_off);
+  *planes_written = true;
+  // pairwise: sums of squared differences, then the thresholds on the host
+  const bool d3 = I.conn == 6;
+  const int zsep = d3 && prm->z_separate ? 1 : 0;
+  if (I.conn == 6) k_pair_sums<6, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, zsep, s.sums);
+  else if (I.conn == 8) k_pair_sums<8, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, 0, s.sums);
+  else k_pair_sums<4, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, 0, s.sums);
+  long long sums[4];
+  long long cent[2][KMAX_CLUSTERS * 3];
+  MODEL_HIP(hipMemcpyAsync(sums, s.sums, sizeof(sums), hipMemcpyDeviceToHost, st));
+  MODEL_HIP(hipMemcpyAsync(cent, s.cent, sizeof(cent), hipMemcpyDeviceToHost, st));
+  MODEL_HIP(hipMemcpyAsync(ctl, s.ctl, 6 * sizeof(int), hipMemcpyDeviceToHost, st));
+  MODEL_HIP(hipStreamSynchronize(st));
+  ST.iters_fg = ctl[4];
+  ST.iters_bg = ctl[5];
+  for (int i = 0; i < prm->k_fg * C; ++i) ST.centroids_fg[i] = cent[0][i];
+  for (int i = 0; i < prm->k_bg * C; ++i) ST.centroids_bg[i] = cent[1][i];
+  ST.pair_sum[0] = sums[0]; ST.pair_count[0] = sums[1];
+  ST.pair_sum[1] = sums[2]; ST.pair_count[1] = sums[3];
+  static_assert(sizeof(unsigned) == 4, "thresholds are 32-bit");
+  unsigned Tab[3 * TMAX];
+  std::memset(Tab, 0, sizeof(Tab));
+  PairParams pp;
+  const double lam[3] = {prm->lambda_xy, prm->lambda_xy / std::sqrt(2.0), prm->lambda_z};
+  const int zg = zsep ? 2 : 0;  // sums of the z table: own group or the x/y one
+  const long long S[3] = {sums[0], sums[0], sums[zg]}, P[3] = {sums[1], sums[1], sums[zg + 1]};
+  for (int t = 0; t < 3; ++t) {
+    pp.n[t] = thresholds(lam[t], S[t], P[t], Tab + t * TMAX);
+    pp.lam[t] = (float)lam[t];
+    pp.inv2s2[t] = (S[t] == 0 || P[t] == 0) ? 0.0f : (float)((double)P[t] / (2.0 * (double)S[t]));
+  }
+  MODEL_HIP(hipMemcpyAsync(s.T, Tab, sizeof(Tab), hipMemcpyHostToDevice, st));
+  if (I.conn == 6)
+    k_pairwise<6, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, I.px, s.T, pp,
+                                              c0 + I.node_off, stride);
+  else if (I.conn == 8)
+    k_pairwise<8, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, I.px, s.T, pp,
+                                              c0 + I.node_off, stride);
+  else
+    k_pairwise<4, BPC><<<rgrid, 256, 0, st>>>(img, C, I.nx, I.ny, I.nz, I.px, s.T, pp,
+                                              c0 + I.node_off, stride);
+  MODEL_HIP(hipGetLastError());
+  MODEL_HIP(hipStreamSynchronize(st));
+  return JFC_OK;
+}
+
+int model_common(jfc_graph *g, int inst, const void *img, int C, int bpc,
+                 const unsigned char *seeds, const jfc_model_params *prm,
+                 jfc_model_stats *stats, void *scratch, int *list, long long cap) {
+  InstDesc I;
+  int *cs = nullptr, *ct = nullptr, *c0 = nullptr;
+  long long stride = 0;
+  int rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride);
+  if (rc) return rc;
+  jfc_model_stats st_local;
+  jfc_model_stats &ST = stats ? *stats : st_local;
+  std::memset(&ST, 0, sizeof(ST));
+  const Scratch s = carve(scratch, list, cap);
+  bool written = false;
+  rc = bpc == 2 ? run_model<2>(I, img, C, seeds, prm, ST, s, cs, ct, c0, stride, graph_stream(g), &written)
+                : run_model<1>(I, img, C, seeds, prm, ST, s, cs, ct, c0, stride, graph_stream(g), &written);
+  if (rc) {
+    if (written) graph_inputs_failed(g, inst);
+    return rc;
+  }
+  graph_mark_inputs(g, inst);
+  return JFC_OK;
+}
+
+}  // namespace
+
 extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
                                int bpc, const uint8_t *seeds, int on_device,
                                const jfc_model_params *prm, jfc_model_stats *stats) {
   int rc = check_model_params(channels, bpc, prm);
   if (rc) return rc;
   InstDesc I;
-  int *cs = nullptr, *ct = nullptr, *c0 = nullptr;
-  long long stride = 0;
-  rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride);
-  if (rc) return rc;
+  int *cs, *ct, *c0;
+  long long stride;
+  if ((rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride))) return rc;
   if (!image || !seeds) return model_fail(JFC_ERR_INVALID, "null argument");
   const long long n = (long long)I.nx * I.ny * I.nz;
   if (n >= (1LL << 31)) return model_fail(JFC_ERR_INVALID, "instance too large for the model path");
-  const int C = channels;
-  const size_t img_bytes = (size_t)n * C * bpc;
-  const bool d3 = I.conn == 6;
-  const int nkeys = bpc == 2 ? 4096 : 255 * C + 1;
-  void *d_img = nullptr;
-  unsigned char *d_seeds = nullptr;
-  int *d_list = nullptr, *d_cnt = nullptr, *d_nT = nullptr;
-  long long *d_kcnt = nullptr, *d_ksum = nullptr, *d_cent = nullptr, *d_acc = nullptr,
-            *d_cf = nullptr, *d_cb = nullptr, *d_sums = nullptr, *d_T = nullptr;
-  jfc_model_stats st_local;
-  jfc_model_stats &ST = stats ? *stats : st_local;
-  memset(&ST, 0, sizeof(ST));
-  std::vector<long long> cent_side[2];
-  long long sums[4] = {0, 0, 0, 0};
-  std::vector<long long> Tab(3 * TMAX, 0);
-  int nT[3] = {0, 0, 0};
   const void *img = image;
   const unsigned char *sd = seeds;
   if (!on_device) {
-    MODEL_HIP(hipMalloc(&d_img, img_bytes));
-    MODEL_HIP(hipMalloc((void **)&d_seeds, n));
-    MODEL_HIP(hipMemcpy(d_img, image, img_bytes, hipMemcpyHostToDevice));
-    MODEL_HIP(hipMemcpy(d_seeds, seeds, n, hipMemcpyHostToDevice));
-    img = d_img;
-    sd = d_seeds;
+    // host inputs cross in one copy each through the graph's staging buffer
+    const size_t ib = align256((size_t)n * channels * bpc);
+    void *stage = nullptr;
+    if ((rc = graph_stage(g, ib + (size_t)n, &stage))) return rc;
+    hipStream_t st = graph_stream(g);
+    MODEL_HIP(hipMemcpyAsync(stage, image, (size_t)n * channels * bpc, hipMemcpyHostToDevice, st));
+    MODEL_HIP(hipMemcpyAsync((unsigned char *)stage + ib, seeds, (size_t)n, hipMemcpyHostToDevice, st));
+    img = stage;
+    sd = (const unsigned char *)stage + ib;
   }
-  MODEL_HIP(hipMalloc((void **)&d_list, n * sizeof(int)));
-  MODEL_HIP(hipMalloc((void **)&d_cnt, sizeof(int)));
-  MODEL_HIP(hipMalloc((void **)&d_kcnt, nkeys * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_ksum, (size_t)nkeys * C * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_cent, KMAX_CLUSTERS * 3 * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_acc, KMAX_CLUSTERS * 4 * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_cf, KMAX_CLUSTERS * 3 * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_cb, KMAX_CLUSTERS * 3 * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_sums, 4 * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_T, 3 * TMAX * sizeof(long long)));
-  MODEL_HIP(hipMalloc((void **)&d_nT, 3 * sizeof(int)));
-
-  for (int side = 1; side <= 2; ++side) {
-    const int K = side == 1 ? prm->k_fg : prm->k_bg;
-    int cnt = 0;
-    MODEL_HIP(hipMemset(d_cnt, 0, sizeof(int)));
-    k_seed_list<<<grid1(n), 256>>>(sd, n, side, d_list, d_cnt);
-    MODEL_HIP(hipMemcpy(&cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost));
-    if (side == 1) ST.seeds_fg = cnt; else ST.seeds_bg = cnt;
-    if (cnt == 0) {
-      rc = model_fail(JFC_ERR_NO_SEEDS, side == 1 ? "no foreground seeds" : "no background seeds");
-      goto done;
-    }
-    // initial centroids: distinct seeded keys, evenly spaced picks
-    // (graphcut_cut_distribution + QGCWidget.cpp:2001-2039)
-    MODEL_HIP(hipMemset(d_kcnt, 0, nkeys * sizeof(long long)));
-    MODEL_HIP(hipMemset(d_ksum, 0, (size_t)nkeys * C * sizeof(long long)));
-    k_key_hist<<<grid1(cnt), 256>>>(img, bpc, C, d_list, d_cnt, d_kcnt, d_ksum);
-    std::vector<long long> kc(nkeys), ks((size_t)nkeys * C);
-    MODEL_HIP(hipMemcpy(kc.data(), d_kcnt, nkeys * sizeof(long long), hipMemcpyDeviceToHost));
-    MODEL_HIP(hipMemcpy(ks.data(), d_ksum, ks.size() * sizeof(long long), hipMemcpyDeviceToHost));
-    std::vector<int> levels;
-    for (int k = 0; k < nkeys; ++k)
-      if (kc[k]) levels.push_back(k);
-    const int m = (int)levels.size();
-    std::vector<long long> cent((size_t)K * C, 0);
-    for (int j = 0; j < K; ++j) {
-      const int pick = K > 1 ? levels[(long long)j * (m - 1) / (K - 1)] : levels[0];
-      const long long nn = kc[pick];
-      for (int c = 0; c < C; ++c)
-        cent[j * C + c] = (ks[(size_t)pick * C + c] * (1 << KM_FRAC) + nn / 2) / nn;
-    }
-    // Lloyd iterations (graphcut_kmeans.cl:21-89 + host update QGCWidget.cpp:2073-2102)
-    int it = 0;
-    while (it < prm->kmeans_iters) {
-      ++it;
-      MODEL_HIP(hipMemcpy(d_cent, cent.data(), cent.size() * sizeof(long long), hipMemcpyHostToDevice));
-      MODEL_HIP(hipMemset(d_acc, 0, KMAX_CLUSTERS * 4 * sizeof(long long)));
-      k_km_assign<<<std::min(grid1(cnt), 1024), 256>>>(img, bpc, C, d_list, d_cnt, d_cent, K, d_acc);
-      std::vector<long long> acc((size_t)K * (C + 1));
-      MODEL_HIP(hipMemcpy(acc.data(), d_acc, acc.size() * sizeof(long long), hipMemcpyDeviceToHost));
-      bool changed = false;
-      for (int k = 0; k < K; ++k) {
-        const long long nk = acc[k * (C + 1) + C];
-        if (!nk) continue;
-        for (int c = 0; c < C; ++c) {
-          const long long nc = (acc[k * (C + 1) + c] * (1 << KM_FRAC) + nk / 2) / nk;
-          if (nc != cent[k * C + c]) { cent[k * C + c] = nc; changed = true; }
-        }
-      }
-      if (!changed) break;
-    }
-    if (side == 1) ST.iters_fg = it; else ST.iters_bg = it;
-    for (int i = 0; i < K * C; ++i)
-      (side == 1 ? ST.centroids_fg : ST.centroids_bg)[i] = cent[i];
-    cent_side[side - 1] = cent;
-  }
-  {
-    // unary terms
-    MODEL_HIP(hipMemcpy(d_cf, cent_side[0].data(), cent_side[0].size() * sizeof(long long), hipMemcpyHostToDevice));
-    MODEL_HIP(hipMemcpy(d_cb, cent_side[1].data(), cent_side[1].size() * sizeof(long long), hipMemcpyHostToDevice));
-    const long long den = prm->range * prm->range * (1LL << (2 * KM_FRAC));
-    k_unary<<<grid1(n), 256>>>(img, bpc, C, n, I.nx, I.px, sd, d_cf, prm->k_fg, d_cb, prm->k_bg, den,
-                               prm->unary_scale, cs + I.node_off, ct + I.node_off);
-    // pairwise terms
-    MODEL_HIP(hipMemset(d_sums, 0, 4 * sizeof(long long)));
-    const int rgrid = (int)std::min<long long>((long long)I.ny * I.nz, 65536);
-    if (d3) k_pair_sums<true><<<rgrid, 256>>>(I, img, bpc, C, prm->z_separate, d_sums);
-    else k_pair_sums<false><<<rgrid, 256>>>(I, img, bpc, C, 0, d_sums);
-    MODEL_HIP(hipMemcpy(sums, d_sums, sizeof(sums), hipMemcpyDeviceToHost));
-    ST.pair_sum[0] = sums[0]; ST.pair_count[0] = sums[1];
-    ST.pair_sum[1] = sums[2]; ST.pair_count[1] = sums[3];
-    nT[0] = thresholds(prm->lambda_xy, sums[0], sums[1], Tab.data());
-    nT[1] = thresholds(prm->lambda_xy / std::sqrt(2.0), sums[0], sums[1], Tab.data() + TMAX);
-    if (d3 && prm->z_separate) nT[2] = thresholds(prm->lambda_z, sums[2], sums[3], Tab.data() + 2 * TMAX);
-    else nT[2] = thresholds(prm->lambda_z, sums[0], sums[1], Tab.data() + 2 * TMAX);
-    MODEL_HIP(hipMemcpy(d_T, Tab.data(), Tab.size() * sizeof(long long), hipMemcpyHostToDevice));
-    MODEL_HIP(hipMemcpy(d_nT, nT, sizeof(nT), hipMemcpyHostToDevice));
-    if (d3) k_pairwise<true><<<rgrid, 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
-    else k_pairwise<false><<<rgrid, 256>>>(I, img, bpc, C, d_T, d_nT, c0, stride);
-    MODEL_HIP(hipDeviceSynchronize());
-    graph_mark_inputs(g, inst);
-  }
-done:
-  if (d_img) (void)hipFree(d_img);
-  if (d_seeds) (void)hipFree(d_seeds);
-  for (void *p : {(void *)d_list, (void *)d_cnt, (void *)d_kcnt, (void *)d_ksum,
-                  (void *)d_cent, (void *)d_acc, (void *)d_cf, (void *)d_cb,
-                  (void *)d_sums, (void *)d_T, (void *)d_nT})
-    if (p) (void)hipFree(p);
-  return rc;
+  // the graph's small scratch block (allocated on its first model call);
+  // the seed list borrows the instance's slice of the excess plane, which
+  // every solve rewrites from the inputs before reading it
+  void *scratch = nullptr;
+  if ((rc = graph_scratch(g, kSmallBytes, &scratch))) return rc;
+  int *list = nullptr;
+  long long cap = 0;
+  if ((rc = graph_work_plane(g, inst, &list, &cap))) return rc;
+  return model_common(g, inst, img, channels, bpc, sd, prm, stats, scratch, list, cap);
 }

@@ -391,14 +391,23 @@ static int row_grid(int nrows) { return nrows < 2048 ? (nrows > 0 ? nrows : 1) :
 // With `active` set it flags the tiles holding excess (`active` zeroed by the
 // launcher); with `zero_h` it starts the labelling at zero heights over all
 // nodes (the reference clears h, QGCWidget.cpp:2400; the push kernel reads
-// the heights of tiles it has not visited in the solve as 0 instead).  It always flags the tiles holding a
-// deficit node in `defi`, the seeds of a global relabel's tile ordering (the
-// seed flags of graphcut_init_global_relabel, graphcut_global_relabel.cl:
-// 19-48, kept current by the push visits instead of a pass over e).
-template <class S>
+// the heights of tiles it has not visited in the solve as 0 instead).  It
+// always flags the tiles holding a deficit node in `defi`, the seeds of a
+// global relabel's tile ordering (the seed flags of
+// graphcut_init_global_relabel, graphcut_global_relabel.cl:19-48, kept
+// current by the push visits instead of a pass over e).
+//
+// The same pass starts the residual-arc masks of the solve (MASKS): 0 leaves
+// them alone, 1 copies mask0 to rmask, 2 builds mask0 from the input planes
+// (residual bit d = c0[d] > 0, every plane fresh) and writes it to both.  One
+// pass instead of a fold, a mask build and a copy: the mask words are
+// written once and the terminal planes read once.
+template <class S, int MASKS>
 __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
                                               unsigned char *active, unsigned char *defi,
-                                              int zero_h) {
+                                              int zero_h, unsigned *mask0) {
+  // planes of eight directions in flight per group: one group per lane
+  constexpr int U = MASKS == 2 ? 1 : RS_U;
   __shared__ long long buf[4];
   long long *dst[3] = {red + RED_CONST, red + RED_DEF0, red + RED_SUP0};
   const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
@@ -407,19 +416,35 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
   for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
     const RowRef<S> R = row_ref<S>(T, r);
     sums.enter(R.inst, dst, stride, buf);
+    const int K = MASKS == 2 ? T.inst[R.inst].K : 0;
     const int ng = row_groups(R.v0, R.nx);
-    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
-      Group g[RS_U];
-      int4 a[RS_U], b[RS_U];
+    for (int k0 = threadIdx.x; k0 < ng; k0 += U * 256) {
+      Group g[U];
+      int4 a[U], b[U], m[U];
 #pragma unroll
-      for (int u = 0; u < RS_U; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int k = k0 + u * 256;
         g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
         a[u] = load4(P.cs, R.v0 + g[u].x, g[u].n);
         b[u] = load4(P.ct, R.v0 + g[u].x, g[u].n);
+        if (MASKS == 1) m[u] = load4((const int *)mask0, R.v0 + g[u].x, g[u].n);
+        if (MASKS == 2) {
+          int4 c[S::KMAX];
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d)
+            c[d] = d < K ? load4(P.c0 + d * P.stride, R.v0 + g[u].x, g[u].n) : make_int4(0, 0, 0, 0);
+          m[u] = make_int4(0, 0, 0, 0);
+#pragma unroll
+          for (int d = 0; d < S::KMAX; ++d) {
+            if (d >= K) break;
+            const int fresh = 1 << (8 + d);
+            m[u].x |= (c[d].x > 0 ? 1 << d : 0) | fresh; m[u].y |= (c[d].y > 0 ? 1 << d : 0) | fresh;
+            m[u].z |= (c[d].z > 0 ? 1 << d : 0) | fresh; m[u].w |= (c[d].w > 0 ? 1 << d : 0) | fresh;
+          }
+        }
       }
 #pragma unroll
-      for (int u = 0; u < RS_U; ++u) {
+      for (int u = 0; u < U; ++u) {
         if (g[u].n == 0) continue;
         const long long v = R.v0 + g[u].x;
         int4 e;
@@ -427,6 +452,8 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
         e.z = a[u].z - b[u].z; e.w = a[u].w - b[u].w;
         store4(P.e, v, g[u].n, e);
         if (zero_h) store4(P.h, v, g[u].n, make_int4(0, 0, 0, 0));
+        if (MASKS) store4((int *)P.rmask, v, g[u].n, m[u]);
+        if (MASKS == 2) store4((int *)mask0, v, g[u].n, m[u]);
         unsigned bits = 0, dbits = 0;
         for (int i = 0; i < g[u].n; ++i) {
           const int s = comp(a[u], i), t = comp(b[u], i);
@@ -445,11 +472,19 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
 }
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
-                 unsigned char *active, unsigned char *defi, bool zero_h, hipStream_t st) {
+                 unsigned char *active, unsigned char *defi, bool zero_h, unsigned *mask0,
+                 int masks, hipStream_t st) {
   if (active) launch_fill_u8(active, T.ntiles, 0, st);
   launch_fill_u8(defi, T.ntiles, 0, st);
-  JFC_DISPATCH(shape, (k_fold<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, red, active, defi,
-                                                                     zero_h ? 1 : 0)));
+  const int z = zero_h ? 1 : 0;
+  const int g = row_grid(T.nrows);
+  if (masks == 2) {
+    JFC_DISPATCH(shape, (k_fold<S, 2><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+  } else if (masks == 1) {
+    JFC_DISPATCH(shape, (k_fold<S, 1><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+  } else {
+    JFC_DISPATCH(shape, (k_fold<S, 0><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+  }
 }
 
 // graphcut_init_cut (graphcut_compute_cut.cl:19-49): label = 1 where
@@ -544,35 +579,6 @@ void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigne
                     hipStream_t st) {
   launch_fill_u8(defi, T.ntiles, 0, st);
   JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, tag, defi)));
-}
-
-// Residual-arc bits of the input planes (see the mask comment below).
-template <class S>
-__global__ __launch_bounds__(256) void k_build_mask(Tiles T, Planes P, unsigned *mask0) {
-  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
-    const RowRef<S> R = row_ref<S>(T, r);
-    const int K = T.inst[R.inst].K;
-    const int ng = row_groups(R.v0, R.nx);
-    for (int k = threadIdx.x; k < ng; k += 256) {
-      const Group g = row_group(R.v0, R.nx, k);
-      const long long v = R.v0 + g.x;
-      int4 m = make_int4(0, 0, 0, 0);
-#pragma unroll
-      for (int d = 0; d < S::KMAX; ++d) {
-        if (d >= K) break;
-        const int4 c = load4(P.c0 + d * P.stride, v, g.n);
-        const int fresh = 1 << (8 + d);
-        m.x |= (c.x > 0 ? 1 << d : 0) | fresh; m.y |= (c.y > 0 ? 1 << d : 0) | fresh;
-        m.z |= (c.z > 0 ? 1 << d : 0) | fresh; m.w |= (c.w > 0 ? 1 << d : 0) | fresh;
-      }
-      store4((int *)mask0, v, g.n, m);
-    }
-  }
-}
-
-void launch_build_mask(int shape, const Tiles &T, const Planes &P, unsigned *mask0,
-                       hipStream_t st) {
-  JFC_DISPATCH(shape, (k_build_mask<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, mask0)));
 }
 
 // Cut energy on the input capacities (QGCWidget.cpp:2452-2520), summed per
@@ -1171,7 +1177,8 @@ static int resident_grid(int want) {
 // written in this solve.  The relabel and cut searches only need the low
 // bits, so they read 4 bytes per node instead of K planes.  mask0 holds the
 // bits of the input planes (every plane fresh), built once per change of the
-// inputs and copied to rmask when a solve starts; k_push reads fresh planes
+// inputs and copied to rmask when a solve starts, both inside the fold pass
+// (k_fold); k_push reads fresh planes
 // from c0 and writes them to cap, so no separate reset copy of the residual
 // planes is needed.
 // Cells of the padded tile a connectivity reads outside the tile: the sides

@@ -44,6 +44,11 @@ int model_fail(int code, const std::string &msg);
 int graph_instance(jfc_graph *g, int inst, InstDesc *I, int **cs, int **ct, int **c0,
                    long long *stride);
 void graph_mark_inputs(jfc_graph *g, int inst);
+void graph_inputs_failed(jfc_graph *g, int inst);
+hipStream_t graph_stream(jfc_graph *g);
+int graph_stage(jfc_graph *g, size_t bytes, void **p);
+int graph_scratch(jfc_graph *g, size_t bytes, void **p);
+int graph_work_plane(jfc_graph *g, int inst, int **plane, long long *n);
 }  // namespace jfc
 
 using namespace jfc;
@@ -332,15 +337,21 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
   const long long n = (long long)I.nx * I.ny * I.nz;
   if (n >= (1LL << 31) - 1) return model_fail(JFC_ERR_INVALID, "instance too large for the model path");
 
+  // Everything runs on the graph's stream.  Host inputs cross in one copy
+  // each through the graph's staging buffer; the small scratch (seed count,
+  // histograms, level representatives, k-means sums, centroids) is the
+  // graph's data-term block; the seed list borrows the instance's slice of
+  // the excess plane, which every solve rewrites from the inputs.
+  const hipStream_t st = graph_stream(g);
   const W4 w = normalize_weight(prm->weight);
   const int k = prm->k;
   const uchar4 *img = reinterpret_cast<const uchar4 *>(rgba);
   const unsigned char *dfg = fg, *dbg = bg;
-  uchar4 *d_img = nullptr;
-  unsigned char *d_fg = nullptr, *d_bg = nullptr;
   int *d_list = nullptr, *d_cnt = nullptr;
   unsigned *d_hist = nullptr, *d_rep = nullptr, *d_acc = nullptr;
   float *d_cent = nullptr;
+  long long list_cap = 0;
+  bool written = false;
   jfc_widget_model_stats st_local;
   jfc_widget_model_stats &ST = stats ? *stats : st_local;
   std::vector<float> cent_used(2 * k * 4, 0.0f);
@@ -348,45 +359,55 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
   memset(&ST, 0, sizeof(ST));
 
   if (!on_device) {
-    WM_HIP(hipMalloc((void **)&d_img, n * sizeof(uchar4)));
-    WM_HIP(hipMalloc((void **)&d_fg, n));
-    WM_HIP(hipMalloc((void **)&d_bg, n));
-    WM_HIP(hipMemcpy(d_img, rgba, n * sizeof(uchar4), hipMemcpyHostToDevice));
-    WM_HIP(hipMemcpy(d_fg, fg, n, hipMemcpyHostToDevice));
-    WM_HIP(hipMemcpy(d_bg, bg, n, hipMemcpyHostToDevice));
-    img = d_img;
-    dfg = d_fg;
-    dbg = d_bg;
+    void *stage = nullptr;
+    if ((rc = graph_stage(g, (size_t)n * 6 + 256, &stage))) return rc;
+    unsigned char *sp = (unsigned char *)stage;
+    WM_HIP(hipMemcpyAsync(sp, rgba, n * sizeof(uchar4), hipMemcpyHostToDevice, st));
+    WM_HIP(hipMemcpyAsync(sp + n * 4, fg, n, hipMemcpyHostToDevice, st));
+    WM_HIP(hipMemcpyAsync(sp + n * 5, bg, n, hipMemcpyHostToDevice, st));
+    img = reinterpret_cast<const uchar4 *>(sp);
+    dfg = sp + n * 4;
+    dbg = sp + n * 5;
   }
-  WM_HIP(hipMalloc((void **)&d_list, n * sizeof(int)));
-  WM_HIP(hipMalloc((void **)&d_cnt, sizeof(int)));
-  WM_HIP(hipMalloc((void **)&d_hist, 2 * kLevels * sizeof(unsigned)));
-  WM_HIP(hipMalloc((void **)&d_rep, kLevels * sizeof(unsigned)));
-  WM_HIP(hipMalloc((void **)&d_acc, (size_t)kMaxK * 4 * kLevels * sizeof(unsigned)));
-  WM_HIP(hipMalloc((void **)&d_cent, 2 * kMaxK * 4 * sizeof(float)));
-  WM_HIP(hipMemset(d_hist, 0, 2 * kLevels * sizeof(unsigned)));
+  {
+    const size_t o_cnt = 0, o_hist = 256, o_rep = o_hist + 2 * kLevels * 4,
+                 o_acc = o_rep + kLevels * 4, o_cent = o_acc + (size_t)kMaxK * 4 * kLevels * 4,
+                 total = o_cent + 2 * kMaxK * 4 * 4;
+    void *sc = nullptr;
+    if ((rc = graph_scratch(g, total, &sc))) return rc;
+    unsigned char *b = (unsigned char *)sc;
+    d_cnt = (int *)(b + o_cnt);
+    d_hist = (unsigned *)(b + o_hist);
+    d_rep = (unsigned *)(b + o_rep);
+    d_acc = (unsigned *)(b + o_acc);
+    d_cent = (float *)(b + o_cent);
+    if ((rc = graph_work_plane(g, inst, &d_list, &list_cap))) return rc;
+  }
+  WM_HIP(hipMemsetAsync(d_hist, 0, 2 * kLevels * sizeof(unsigned), st));
 
   for (int side = 0; side < 2; ++side) {
     int cnt = 0;
-    WM_HIP(hipMemset(d_cnt, 0, sizeof(int)));
-    k_wm_seed_list<<<grid1(n), 256>>>(side ? dbg : dfg, n, d_list, d_cnt);
+    WM_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int), st));
+    k_wm_seed_list<<<grid1(n), 256, 0, st>>>(side ? dbg : dfg, n, d_list, d_cnt);
     WM_HIP(hipGetLastError());
-    WM_HIP(hipMemcpy(&cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost));
+    WM_HIP(hipMemcpyAsync(&cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    WM_HIP(hipStreamSynchronize(st));
     ST.seeds[side] = cnt;
     if (cnt == 0) {
       // computeBFS / kmeans "no labels." (QGCW:2015-2016)
       rc = model_fail(JFC_ERR_NO_SEEDS, side ? "no background seeds" : "no foreground seeds");
       goto done;
     }
-    WM_HIP(hipMemset(d_rep, 0, kLevels * sizeof(unsigned)));
-    k_wm_levels<<<grid1(cnt), 256>>>(img, d_list, d_cnt, w, d_hist + side * kLevels, d_rep);
+    WM_HIP(hipMemsetAsync(d_rep, 0, kLevels * sizeof(unsigned), st));
+    k_wm_levels<<<grid1(cnt), 256, 0, st>>>(img, d_list, d_cnt, w, d_hist + side * kLevels, d_rep);
     WM_HIP(hipGetLastError());
     if (!prm->lazy) continue;
 
     // initial centroids: the representatives of the seeded levels, picked at
     // (count - 1) * i / (k - 1) (QGCW:2011-2039), scaled by weight / 255
     std::vector<unsigned> rep(kLevels);
-    WM_HIP(hipMemcpy(rep.data(), d_rep, kLevels * sizeof(unsigned), hipMemcpyDeviceToHost));
+    WM_HIP(hipMemcpyAsync(rep.data(), d_rep, kLevels * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    WM_HIP(hipStreamSynchronize(st));
     std::vector<unsigned> dist;
     for (int l = 0; l < kLevels; ++l)
       if (rep[l]) dist.push_back(rep[l]);
@@ -394,10 +415,14 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
     float scale[4];
     for (int c = 0; c < 4; ++c) scale[c] = w.v[c] / 255.0f;
     std::vector<float> cent(k * 4), prev(k * 4);
+    std::vector<uchar4> picked(k);
     for (int i = 0; i < k; ++i) {
       const long long idx = (long long)dist[dsize * (unsigned)i / (unsigned)(k - 1)] - 1;
-      uchar4 t;
-      WM_HIP(hipMemcpy(&t, img + idx, sizeof(uchar4), hipMemcpyDeviceToHost));
+      WM_HIP(hipMemcpyAsync(&picked[i], img + idx, sizeof(uchar4), hipMemcpyDeviceToHost, st));
+    }
+    WM_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < k; ++i) {
+      const uchar4 t = picked[i];
       const unsigned tb[4] = {t.x, t.y, t.z, t.w};
       for (int c = 0; c < 4; ++c) cent[i * 4 + c] = (float)tb[c] * scale[c];
     }
@@ -411,11 +436,12 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
     int it = 0;
     for (; it < prm->kmeans_iterations;) {
       prev = cent;
-      WM_HIP(hipMemcpy(d_cent, prev.data(), k * 4 * sizeof(float), hipMemcpyHostToDevice));
-      WM_HIP(hipMemset(d_acc, 0, acc.size() * sizeof(unsigned)));
-      k_wm_assign<<<std::min(grid1(cnt), 1024), 256>>>(img, d_list, d_cnt, w, d_cent, k, d_acc);
+      WM_HIP(hipMemcpyAsync(d_cent, prev.data(), k * 4 * sizeof(float), hipMemcpyHostToDevice, st));
+      WM_HIP(hipMemsetAsync(d_acc, 0, acc.size() * sizeof(unsigned), st));
+      k_wm_assign<<<std::min(grid1(cnt), 1024), 256, 0, st>>>(img, d_list, d_cnt, w, d_cent, k, d_acc);
       WM_HIP(hipGetLastError());
-      WM_HIP(hipMemcpy(acc.data(), d_acc, acc.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+      WM_HIP(hipMemcpyAsync(acc.data(), d_acc, acc.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      WM_HIP(hipStreamSynchronize(st));
       ++it;
       float err = 0.0f;
       for (int i = 0; i < k; ++i) {
@@ -448,42 +474,43 @@ extern "C" int jfc_model_widget(jfc_graph *g, int inst, const uint8_t *rgba, con
     }
   }
   {
-    WM_HIP(hipMemcpy(hist.data(), d_hist, hist.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+    WM_HIP(hipMemcpyAsync(hist.data(), d_hist, hist.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    WM_HIP(hipStreamSynchronize(st));
     for (int side = 0; side < 2; ++side)
       for (int l = 0; l < kLevels; ++l) ST.hist_sum[side] += hist[side * kLevels + l];
     const int shift = d3 ? 3 : 2;
+    written = true;
     if (prm->lazy) {
-      WM_HIP(hipMemcpy(d_cent, cent_used.data(), cent_used.size() * sizeof(float),
-                       hipMemcpyHostToDevice));
-      k_wm_unary<true><<<grid1(n), 256>>>(img, dfg, dbg, n, w, d_cent, k, nullptr, 0, 0, shift,
-                                          prm->lazy_terminal, I.nx, I.px, cs + I.node_off,
-                                          ct + I.node_off);
+      WM_HIP(hipMemcpyAsync(d_cent, cent_used.data(), cent_used.size() * sizeof(float),
+                            hipMemcpyHostToDevice, st));
+      k_wm_unary<true><<<grid1(n), 256, 0, st>>>(img, dfg, dbg, n, w, d_cent, k, nullptr, 0, 0,
+                                                 shift, prm->lazy_terminal, I.nx, I.px,
+                                                 cs + I.node_off, ct + I.node_off);
     } else {
-      k_wm_unary<false><<<grid1(n), 256>>>(img, dfg, dbg, n, w, nullptr, 0, d_hist,
-                                           ST.hist_sum[0], ST.hist_sum[1], shift,
-                                           prm->bk_terminal, I.nx, I.px, cs + I.node_off,
-                                           ct + I.node_off);
+      k_wm_unary<false><<<grid1(n), 256, 0, st>>>(img, dfg, dbg, n, w, nullptr, 0, d_hist,
+                                                  ST.hist_sum[0], ST.hist_sum[1], shift,
+                                                  prm->bk_terminal, I.nx, I.px, cs + I.node_off,
+                                                  ct + I.node_off);
     }
     WM_HIP(hipGetLastError());
     const int grid = (int)std::min<long long>((long long)I.ny * I.nz, 65536);
     if (prm->lazy) {
       // lambda / SHRT_MAX (graphcut_compute_capacity.cl:82)
       const float scale = prm->lazy_neighbor / (float)SHRT_MAX;
-      if (d3) k_wm_pairwise<true, true><<<grid, 256>>>(I, img, w, scale, c0, stride);
-      else k_wm_pairwise<true, false><<<grid, 256>>>(I, img, w, scale, c0, stride);
+      if (d3) k_wm_pairwise<true, true><<<grid, 256, 0, st>>>(I, img, w, scale, c0, stride);
+      else k_wm_pairwise<true, false><<<grid, 256, 0, st>>>(I, img, w, scale, c0, stride);
     } else {
       // 1 / (SHRT_MAX * sigma) (graphcut_compute_capacity.cl:29)
       const float scale = 1.0f / ((float)SHRT_MAX * prm->bk_neighbor);
-      if (d3) k_wm_pairwise<false, true><<<grid, 256>>>(I, img, w, scale, c0, stride);
-      else k_wm_pairwise<false, false><<<grid, 256>>>(I, img, w, scale, c0, stride);
+      if (d3) k_wm_pairwise<false, true><<<grid, 256, 0, st>>>(I, img, w, scale, c0, stride);
+      else k_wm_pairwise<false, false><<<grid, 256, 0, st>>>(I, img, w, scale, c0, stride);
     }
     WM_HIP(hipGetLastError());
-    WM_HIP(hipDeviceSynchronize());
+    WM_HIP(hipStreamSynchronize(st));
     graph_mark_inputs(g, inst);
   }
 done:
-  for (void *p : {(void *)d_img, (void *)d_fg, (void *)d_bg, (void *)d_list, (void *)d_cnt,
-                  (void *)d_hist, (void *)d_rep, (void *)d_acc, (void *)d_cent})
-    if (p) (void)hipFree(p);
+  if (rc && written) graph_inputs_failed(g, inst);
+  (void)list_cap;
   return rc;
 }

@@ -346,6 +346,13 @@ def test_input_validation_gpu(gpu):
     with pytest.raises(gpu.JFCError) as e:
         g.set_neighbor_caps(0, -np.ones(256, np.int32))
     assert e.value.code == gpu.ERR_OVERFLOW
+    # a setter that failed zeroed its plane: the solve refuses the instance
+    # until its capacities are set again
+    with pytest.raises(gpu.JFCError) as e:
+        g.solve()
+    assert e.value.code == gpu.ERR_STATE
+    g.set_terminal_caps(np.zeros(256, np.int32), np.zeros(256, np.int32))
+    g.set_neighbor_caps(0, np.ones(256, np.int32))
     with pytest.raises(gpu.JFCError) as e:
         g.solve()  # no terminal capacity anywhere: the reference's "no labels"
     assert e.value.code == gpu.ERR_NO_SEEDS
