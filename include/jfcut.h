@@ -67,8 +67,11 @@ typedef struct jfc_graph jfc_graph;
 typedef struct {
   int max_sweeps;         /* totalIterations: cap on push-relabel sweeps (each sweep
                              visits every colour class of active tiles once) */
-  int check_every;        /* countIterations: sweeps between device->host
-                             convergence checks */
+  int check_every;        /* countIterations: sweeps queued ahead of the host's
+                             convergence check (default 2, at most 7).  The
+                             device decides after every sweep whether to go on
+                             (no active tile, relabel due, sweep budget
+                             spent); sweeps queued past that point do nothing */
   double relabel_work;    /* run an exact global relabel once the tiles visited
                              since the last one exceed this many times the tile
                              count (0 = only at start and on certificate failure) */
@@ -101,6 +104,12 @@ typedef struct {
                              QGCW:2419-2427): Jacobi relabel plus line pushes
                              along each axis, 4- and 6-connected graphs only.
                              Same flow and labels either way. */
+  int sparse_local_iters; /* cap on LDS discharge iterations per visit in a
+                             push launch whose active-tile list is no longer
+                             than its grid (each workgroup visits at most one
+                             tile): 0 = 64 (default), negative = max_local_iters
+                             (the reference loops each visit until no push,
+                             graphcut_push_relabel.cl:77-182) */
 } jfc_options;
 
 #define JFC_METHOD_JFCUT 0

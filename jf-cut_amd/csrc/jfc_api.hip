@@ -104,6 +104,9 @@ int log2_round(double v) { return v <= 1.0 ? 0 : (int)std::lround(std::log2(v));
 // single address takes one atomic per workgroup).
 constexpr int kIterSlots = 5, kTotals = kIterSlots + 64;
 
+// Sweeps whose stop words can be in flight at once (pinned words and events).
+constexpr int kStopRing = 8;
+
 // algorithmic HBM bytes per node of one push-relabel visit: read and write
 // e, h and K residual planes (int32)
 long long push_bytes_per_node(int K) { return 2LL * 4 * (2 + K); }
@@ -182,15 +185,23 @@ struct jfc_graph {
     int recent[3];    // active tiles of the latest sweeps
     int flag;         // certificate word of a cut search
     int bad;          // validation word of a setter
+    int sweeps;       // sweeps run (the device's count)
+    int stopw[kStopRing];  // stop words of the sweeps in flight
     long long red[RED_SLOTS];  // one instance's sums (energy)
   } *pin = nullptr;
   // Pinned copies of a solve's results: per-instance sums (RED_SLOTS each),
   // remaining deficits, then the kTotals counters.
   long long *pin_res = nullptr;
   hipEvent_t cnt_ev[2] = {nullptr, nullptr};
+  hipEvent_t stop_ev[kStopRing] = {};
   int ensure_pin() {
     if (pin) return JFC_OK;
     for (hipEvent_t &ev : cnt_ev)
+      if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
+        return fail(JFC_ERR_DEVICE, "event creation failed");
+      }
+    for (hipEvent_t &ev : stop_ev)
       if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
         ev = nullptr;
         return fail(JFC_ERR_DEVICE, "event creation failed");
@@ -211,6 +222,9 @@ struct jfc_graph {
     hipGraphExec_t exec = nullptr;
     hipStream_t st = nullptr;
     int mli = 0;
+    int sparse = 0;
+    int max_sweeps = 0;
+    double stall = 0.0, work = 0.0;
     const int *hist = nullptr;
   } sweep_graph[2];
 
@@ -247,6 +261,10 @@ struct jfc_graph {
     priv_event = nullptr;
     priv_stream = nullptr;
     for (hipEvent_t &ev : cnt_ev) {
+      if (ev) (void)hipEventDestroy(ev);
+      ev = nullptr;
+    }
+    for (hipEvent_t &ev : stop_ev) {
       if (ev) (void)hipEventDestroy(ev);
       ev = nullptr;
     }
@@ -439,7 +457,7 @@ extern "C" void jfc_default_options(jfc_options *o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->max_sweeps = 1 << 20;
-  o->check_every = 1;
+  o->check_every = 2;
   o->relabel_work = 2.0;
   o->max_local_iters = 0;
   o->tile_shape = JFC_TILE_DEFAULT;
@@ -508,7 +526,7 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->d_total, kTotals, "counters")) ||
       (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
       (rc = dalloc(&g->d_energy, (size_t)count, "reductions")) ||
-      (rc = dalloc(&g->d_sweep_state, 2, "sweep counters")) ||
+      (rc = dalloc(&g->d_sweep_state, kSweepState, "sweep counters")) ||
       (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
       (rc = dalloc(&g->d_flagword, 4, "flags"))) {
     delete g;
@@ -869,6 +887,18 @@ struct Solver {
   Timer tm;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> push_ev, gr_ev, cut_ev;
 
+  SweepCtl sweep_ctl() const {
+    SweepCtl c;
+    c.hist = g->d_sweep;
+    c.state = g->d_sweep_state;
+    c.total = g->d_total;
+    c.cap = g->sweep_cap;
+    c.max_sweeps = o.max_sweeps;
+    c.stall_ratio = o.stall_ratio;
+    c.budget = o.relabel_work > 0 ? (long long)std::floor(o.relabel_work * g->ntiles) : 0;
+    return c;
+  }
+
   Queue queue_from_list() {
     Queue q{g->d_list[g->slot], g->d_ctr, g->slot, -1};
     return q;
@@ -909,7 +939,7 @@ struct Solver {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
-      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, q, nullptr, total, st);
+      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, q, nullptr, total, nullptr, st);
       flood(q, false);
     } else {
       flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles}, true);
@@ -927,7 +957,7 @@ struct Solver {
       for (int j = 0; j < kPassGroup; ++j, ++pass) {
         Queue qc = queue_from_list();
         launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, qc, nullptr, total,
-                       st);
+                       nullptr, st);
         flood(qc, false);
         advance();
         *passes += 1;
@@ -1008,14 +1038,15 @@ struct Solver {
       Queue q = queue_from_list();
       const int n = g->color_off[c + 1] - g->color_off[c];
       launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, q,
-                     g->d_sweep_state, g->d_total, s_);
+                     g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_);
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
-                  g->d_defi, g->d_hmat, o.max_local_iters, g->d_total + kIterSlots, s_);
+                  g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,
+                  g->d_total + kIterSlots, s_);
       if (timed && o.profile) push_ev.push_back({a, tm.mark()});
       advance();
     }
-    launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, s_);
+    launch_sweep_end(sweep_ctl(), s_);
   }
 
   // One Fast-Cut sweep (QGCWidget::relabelPushFast, QGCW:2156-2205): every
@@ -1023,13 +1054,13 @@ struct Solver {
   void issue_fast_sweep() {
     Queue q = queue_from_list();
     launch_compact(g->d_color_tiles, g->ntiles, 0, g->d_flag, nullptr, q, g->d_sweep_state,
-                   g->d_total, st);
+                   g->d_total, g->d_sweep_state + 2, st);
     hipEvent_t a = o.profile ? tm.mark() : nullptr;
     launch_fc_sweep(g->shape, std::min(g->nwg, g->ntiles), g->tiles(), g->planes(), q,
                     g->d_flag, g->d_defi, st);
     if (o.profile) push_ev.push_back({a, tm.mark()});
     advance();
-    launch_sweep_end(g->d_sweep, g->d_sweep_state, g->sweep_cap, st);
+    launch_sweep_end(sweep_ctl(), st);
   }
 
   // Launch one sweep as a hipGraph captured on first use (per starting
@@ -1037,7 +1068,10 @@ struct Solver {
   // the captured graphs).  The sweep contains no memset node.
   int replay_sweep() {
     auto &sg = g->sweep_graph[g->slot];
-    if (!sg.exec || sg.st != st || sg.mli != o.max_local_iters || sg.hist != g->d_sweep) {
+    if (!sg.exec || sg.st != st || sg.mli != o.max_local_iters ||
+        sg.sparse != o.sparse_local_iters || sg.hist != g->d_sweep ||
+        sg.max_sweeps != o.max_sweeps || sg.stall != o.stall_ratio ||
+        sg.work != o.relabel_work) {
       if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
       sg = jfc_graph::SweepGraph{};
       const int slot0 = g->slot;
@@ -1055,6 +1089,10 @@ struct Solver {
       }
       sg.st = st;
       sg.mli = o.max_local_iters;
+      sg.sparse = o.sparse_local_iters;
+      sg.max_sweeps = o.max_sweeps;
+      sg.stall = o.stall_ratio;
+      sg.work = o.relabel_work;
       sg.hist = g->d_sweep;
     }
     JFC_HIP(hipGraphLaunch(sg.exec, st));
@@ -1084,7 +1122,7 @@ struct Solver {
       f.add(g->d_ctr, 2 * QSLOT, 0, 4);
       f.add(g->d_red, 2LL * nI * RED_SLOTS, 0, 4);
       f.add(g->d_total, 2 * kTotals, 0, 4);
-      f.add(g->d_sweep_state, 2, 0, 4);
+      f.add(g->d_sweep_state, kSweepState, 0, 4);
       f.add(g->d_mat, g->ntiles, 0, 1);
       f.add(g->d_hmat, g->ntiles, 0, 1);
       launch_fill_multi(f, st);
@@ -1116,71 +1154,65 @@ struct Solver {
     if (o.initial_relabel) {
       if ((rc = global_relabel())) return rc;
     }  // else the zero heights (a valid labelling) and the fold's flags
-    long long total_at_gr = 0;
-    int since_check = 0;
+    // Sweeps go out up to `ahead` beyond the last one whose stop word the
+    // host has read.  The device decides after every sweep whether to go on
+    // (k_sweep_end: no active tile, relabel due, budget spent), and a sweep
+    // queued after a stop compacts nothing, so the host's wait for a stop
+    // word overlaps the sweeps queued behind it.
+    const int ahead = std::min(std::max(1, o.check_every), kStopRing - 1);
     int cert_rounds = 0;
     bool done = false;
-    int sweep = 0;
-    int sweep_at_gr = 0;
-    std::vector<int> recent;
-    while (!done) {
-      bool active = true;
-      while (sweep < o.max_sweeps) {
-        // the first sweep of a solve runs eagerly (it also primes the
-        // launchers' occupancy queries); later ones replay a captured graph
-        if (o.method == JFC_METHOD_FASTCUT) {
-          issue_fast_sweep();
-          s.push_launches += 2 + 2 * g->dim;
-        } else {
-          if (o.graph_sweeps && !o.profile && sweep > 0) {
-            if ((rc = replay_sweep())) return rc;
+    int issued = 0;  // sweeps issued in this solve, no-ops included
+    for (;;) {
+      int stopw = 0;
+      int sent = 0, known = 0;
+      for (;;) {
+        while (sent < known + ahead) {
+          // the first sweep of a solve runs eagerly (it also primes the
+          // launchers' occupancy queries); later ones replay a captured graph
+          if (o.method == JFC_METHOD_FASTCUT) {
+            issue_fast_sweep();
+            s.push_launches += 2 + 2 * g->dim;
           } else {
-            issue_sweep(st, true);
+            if (o.graph_sweeps && !o.profile && issued > 0) {
+              if ((rc = replay_sweep())) return rc;
+            } else {
+              issue_sweep(st, true);
+            }
+            s.push_launches += g->ncolors;
           }
-          s.push_launches += g->ncolors;
-        }
-        ++sweep;
-        if (++since_check >= o.check_every || sweep == o.max_sweeps) {
-          since_check = 0;
-          // active-tile counts of the sweeps since the last global relabel
-          const int first = std::max(sweep_at_gr, sweep - 2);
-          recent.assign(sweep - first, 0);
-          JFC_HIP(hipMemcpyAsync(g->pin->recent, g->d_sweep + first, recent.size() * sizeof(int),
+          const int k = sent % kStopRing;
+          JFC_HIP(hipMemcpyAsync(&g->pin->stopw[k], g->d_sweep_state + 2, sizeof(int),
                                  hipMemcpyDeviceToHost, st));
-          JFC_HIP(hipMemcpyAsync(&g->pin->total, g->d_total, sizeof(long long),
-                                 hipMemcpyDeviceToHost, st));
-          JFC_HIP(hipEventRecord(g->cnt_ev[0], st));
-          JFC_HIP(hipEventSynchronize(g->cnt_ev[0]));
-          for (size_t i = 0; i < recent.size(); ++i) recent[i] = g->pin->recent[i];
-          const long long total = g->pin->total;
-          const int cnt = recent.back();
-          if (cnt == 0) { active = false; break; }
-          // relabel when the active set stops shrinking (nodes cut off from
-          // every deficit keep sloshing until their heights are recomputed),
-          // or when the work since the last relabel exceeds the budget
-          // stalled: the active set neither shrinks nor grows by more than
-          // (1 - stall_ratio); a growing set is flow still spreading
-          const double prev = recent.size() >= 2 ? recent[recent.size() - 2] : 0.0;
-          const bool stalled = o.stall_ratio > 0 && recent.size() >= 2 &&
-                               cnt >= o.stall_ratio * prev && cnt * o.stall_ratio <= prev;
-          const bool budget = o.relabel_work > 0 &&
-                              (double)(total - total_at_gr) > o.relabel_work * g->ntiles;
-          if (stalled || budget) {
-            if ((rc = global_relabel())) return rc;
-            total_at_gr = total;
-            sweep_at_gr = sweep;
-          }
+          JFC_HIP(hipEventRecord(g->stop_ev[k], st));
+          ++sent;
+          ++issued;
         }
+        const int k = known % kStopRing;
+        JFC_HIP(hipEventSynchronize(g->stop_ev[k]));
+        stopw = g->pin->stopw[k];
+        ++known;
+        if (stopw) break;
       }
-      s.sweeps = sweep;
-      if (active) break;  // sweep budget exhausted
+      // the sweeps queued behind the stop were no-ops
+      if (stopw == 3) break;  // sweep budget spent
+      if (stopw == 2) {
+        if ((rc = global_relabel())) return rc;
+        launch_sweep_resume(sweep_ctl(), st);
+        continue;
+      }
+      // no active tile: the canonical cut doubles as the certificate
       int found = 0;
       if ((rc = canonical_cut(&found))) return rc;
-      if (!found) { done = true; break; }
+      if (!found) {
+        done = true;
+        break;
+      }
       // an augmenting path survived: relabel exactly and continue
       s.certificate_failures++;
       if (++cert_rounds > 16) return fail(JFC_ERR_STATE, "certificate kept failing");
       if ((rc = global_relabel())) return rc;
+      launch_sweep_resume(sweep_ctl(), st);
     }
     if (!done) {
       // labels of the current preflow are still useful to callers
@@ -1196,7 +1228,10 @@ struct Solver {
     JFC_HIP(hipMemcpyAsync(pdef, g->d_def, nI * sizeof(long long), hipMemcpyDeviceToHost, st));
     JFC_HIP(hipMemcpyAsync(totals, g->d_total, kTotals * sizeof(long long),
                            hipMemcpyDeviceToHost, st));
+    JFC_HIP(hipMemcpyAsync(&g->pin->sweeps, g->d_sweep_state + 1, sizeof(int),
+                           hipMemcpyDeviceToHost, st));
     JFC_HIP(hipStreamSynchronize(st));
+    s.sweeps = g->pin->sweeps;
     std::copy(pred, pred + red.size(), red.begin());
     const std::vector<long long> def(pdef, pdef + nI);
     for (int i = 0; i < nI; ++i)
@@ -1211,7 +1246,7 @@ struct Solver {
     s.push_tiles = totals[0];
     s.local_iters = 0;
     for (int k = 0; k < 64; ++k) s.local_iters += totals[kIterSlots + k];
-    g->last_sweeps = sweep;
+    g->last_sweeps = s.sweeps;
     // a relabel's first pass visits every tile without a compaction
     s.relabel_tiles = totals[1] + (long long)g->ntiles * s.global_relabels;
     s.cut_tiles = totals[2];
@@ -1247,6 +1282,9 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.check_every <= 0) S.o.check_every = 1;
   // 0: the cap jfc_autotune chose for the graph's size class, else 2
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
+  // 0: 64; negative: the dense cap
+  if (S.o.sparse_local_iters == 0) S.o.sparse_local_iters = 64;
+  if (S.o.sparse_local_iters < 0) S.o.sparse_local_iters = S.o.max_local_iters;
   // -1: the choice jfc_autotune made for the size class, else none
   if (S.o.initial_relabel < 0) S.o.initial_relabel = g->tuned_ir > 0 ? 1 : 0;
   if (S.o.tile_order != 0 && S.o.tile_order != 1)

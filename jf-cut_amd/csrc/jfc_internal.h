@@ -155,26 +155,42 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *order, int max_tile_dim, int *max_dist,
                        const TileLines &lines, int jump_flooding, hipStream_t st);
 
-// state = {count of the running sweep, sweep index}: hist[index] = count,
-// then the next sweep starts at 0.
-void launch_sweep_end(int *hist, int *state, int cap, hipStream_t st);
+// Sweep bookkeeping on the device.  state (8 ints): [0] active tiles of the
+// running sweep, [1] sweep index, [2] stop word (0 sweep on, 1 no active
+// tile, 2 relabel due, 3 sweep budget spent), [3] sweep index of the last
+// relabel, [4..5] tile visits (int64) at the last relabel.
+struct SweepCtl {
+  int *hist;             // active tiles per sweep
+  int *state;
+  const long long *total;  // tile visits of the push kernel so far
+  int cap;               // entries of hist
+  int max_sweeps;
+  double stall_ratio;
+  long long budget;      // tile visits between relabels (0 = no budget)
+};
+constexpr int kSweepState = 8;
+void launch_sweep_end(const SweepCtl &c, hipStream_t st);
+void launch_sweep_resume(const SweepCtl &c, hipStream_t st);
 
 // Queue list entries: tile index, plus kEntryMat when compacted with `mat`
 // and mat[t] is set.
 constexpr int kEntryMat = 1 << 30;
+// stop (or null): a nonzero word makes the compaction a no-op.
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const Queue &q, int *sweep_cnt,
-                    long long *total, hipStream_t st);
+                    long long *total, const int *stop, hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in
 // this solve (cleared when a solve starts).  defi: a visit rewrites its
 // tile's deficit flag (pushes never create deficits, so unvisited tiles'
 // flags stay a superset).  hmat: per tile, 1 once its heights are stored in
 // this solve (by a push visit, which then writes all of them, or by a global
 // relabel); heights of a tile with hmat 0 read as 0, the labelling a solve
-// starts from.  iter_total: 64 int64 counters of local iterations (or null).
+// starts from.  max_iters caps the local iterations of a visit, sparse_iters
+// those of a launch whose list fits its grid.  iter_total: 64 int64 counters
+// of local iterations (or null).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, unsigned char *hmat, int max_iters,
+                 unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
                  long long *iter_total, hipStream_t st);
 // Fast-Cut comparator (jfc_fastcut.hip)
 void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);

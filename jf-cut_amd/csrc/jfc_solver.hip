@@ -1004,9 +1004,14 @@ constexpr int kCompactBlock = 1024;
 __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n, int reverse,
                                                            unsigned char *flag,
                                                            const unsigned char *mat, Queue q,
-                                                           int *sweep_cnt, long long *total) {
+                                                           int *sweep_cnt, long long *total,
+                                                           const int *stop) {
   __shared__ int s_cnt[kCompactBlock / 64];
   __shared__ int s_base;
+  // a sweep issued after the solve's decision to stop sweeping (a relabel is
+  // due, no tile is active, or the sweep budget is spent) compacts nothing
+  // and leaves the flags for the sweeps after the host has acted
+  if (stop && *stop) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int t = -1;
   bool f = false;
@@ -1039,24 +1044,62 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
 }
 
 // End of a sweep: file the sweep's active-tile count (accumulated by the
-// compactions in state[0]) under the sweep index state[1] and start the next.
-__global__ void k_sweep_end(int *hist, int *state, int cap) {
-  const int i = state[1];
-  if (i < cap) hist[i] = state[0];
-  state[0] = 0;
-  state[1] = i + 1;
+// compactions in c.state[0]) under the sweep index c.state[1], start the next
+// sweep, and decide on the device what the host would decide after reading
+// the count, so that the host can queue sweeps ahead of its checks:
+//   stop = 1: no tile was active (converged, or the certificate decides);
+//   stop = 2: a global relabel is due -- the active set neither shrank nor
+//             grew by more than (1 - stall_ratio) over the last sweep since
+//             the last relabel, or the tile visits since it exceed the
+//             budget (the work of relabel_work visits per tile);
+//   stop = 3: the sweep budget is spent.
+// A sweep that starts with stop set compacts nothing and files nothing.
+__global__ void k_sweep_end(SweepCtl c) {
+  int *st = c.state;
+  if (st[2]) return;
+  const int i = st[1];
+  const int cnt = st[0];
+  if (i < c.cap) c.hist[i] = cnt;
+  st[0] = 0;
+  st[1] = i + 1;
+  int stop = 0;
+  if (cnt == 0) {
+    stop = 1;
+  } else if (i + 1 >= c.max_sweeps) {
+    stop = 3;
+  } else {
+    const int since = i - st[3];  // sweeps since the last relabel, before this one
+    const int prev = since >= 1 && i >= 1 ? c.hist[i - 1] : 0;
+    const bool stalled = c.stall_ratio > 0.0 && since >= 1 &&
+                         (double)cnt >= c.stall_ratio * prev && cnt * c.stall_ratio <= (double)prev;
+    const long long done = c.total[0] - *(const long long *)(st + 4);
+    const bool budget = c.budget > 0 && (double)done > (double)c.budget;
+    if (stalled || budget) stop = 2;
+  }
+  st[2] = stop;
 }
 
-void launch_sweep_end(int *hist, int *state, int cap, hipStream_t st) {
-  k_sweep_end<<<1, 1, 0, st>>>(hist, state, cap);
+void launch_sweep_end(const SweepCtl &c, hipStream_t st) { k_sweep_end<<<1, 1, 0, st>>>(c); }
+
+// After a global relabel: sweeping resumes, and the relabel's sweep index
+// and visit total are the base of the next stall and budget tests.
+__global__ void k_sweep_resume(SweepCtl c) {
+  int *st = c.state;
+  st[2] = 0;
+  st[3] = st[1];
+  *(long long *)(st + 4) = c.total[0];
+}
+
+void launch_sweep_resume(const SweepCtl &c, hipStream_t st) {
+  k_sweep_resume<<<1, 1, 0, st>>>(c);
 }
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const Queue &q, int *sweep_cnt,
-                    long long *total, hipStream_t st) {
+                    long long *total, const int *stop, hipStream_t st) {
   if (n <= 0) return;
   k_compact<<<(n + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, st>>>(
-      src, n, reverse, flag, mat, q, sweep_cnt, total);
+      src, n, reverse, flag, mat, q, sweep_cnt, total, stop);
 }
 
 // ---- persistent tile queue ----------------------------------------------------------
@@ -1244,7 +1287,10 @@ constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 // only excess needs LDS atomics.  Flow pushed into halo cells accumulates in
 // their LDS slots and is flushed to HBM once per visit.  The loop ends when a
 // whole iteration pushes nothing (graphcut_push_relabel.cl:77-182) or after
-// max_iters iterations.
+// max_iters iterations -- sparse_iters when the launch's list is no longer
+// than its grid (every workgroup holds at most one tile, so a visit that
+// keeps iterating delays no other tile, and the flow crosses the tile in one
+// visit instead of one sweep per step).
 template <class S>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
 // without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
@@ -1252,7 +1298,8 @@ template <class S>
 __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
                                               unsigned char *defi, unsigned char *hmat,
-                                              int max_iters, long long *iter_total) {
+                                              int max_iters, int sparse_iters,
+                                              long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -1262,6 +1309,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
   __shared__ int s_w[8];
   int par = 0;
   QueueCursor Qc(q, &s_run);
+  const int cap_iters = Qc.count <= (int)gridDim.x ? sparse_iters : max_iters;
   const long long st = P.stride;
   long long my_iters = 0;
   int idx = -1, end = 0;
@@ -1418,7 +1466,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
         }
       }
       any = block_any(pushed, s_w, par);
-    } while (any && ++it < max_iters);
+    } while (any && ++it < cap_iters);
     my_iters += it + (any ? 0 : 1);
 
     // ---- write back what changed (stores only)
@@ -1500,10 +1548,10 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, unsigned char *hmat, int max_iters,
+                 unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
                  long long *iter_total, hipStream_t st) {
   JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
-                          T, P, q, flag, mat, defi, hmat, max_iters, iter_total)));
+                          T, P, q, flag, mat, defi, hmat, max_iters, sparse_iters, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------

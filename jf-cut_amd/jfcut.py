@@ -38,7 +38,8 @@ class Options(ctypes.Structure):
                 ("tile_shape", ctypes.c_int), ("profile", ctypes.c_int),
                 ("stream", ctypes.c_void_p), ("initial_relabel", ctypes.c_int),
                 ("stall_ratio", ctypes.c_double), ("graph_sweeps", ctypes.c_int),
-                ("tile_order", ctypes.c_int), ("method", ctypes.c_int)]
+                ("tile_order", ctypes.c_int), ("method", ctypes.c_int),
+                ("sparse_local_iters", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
