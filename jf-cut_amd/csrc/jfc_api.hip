@@ -149,6 +149,7 @@ struct jfc_graph {
   int gr_tag = 1;                   // tag of the last global relabel's heights
   unsigned char *d_mat = nullptr;  // per tile: residual planes written this solve
   unsigned char *d_hmat = nullptr; // per tile: heights stored this solve
+  unsigned char *d_touched = nullptr;  // per tile: planes or masks written by a flush
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
   long long *d_red = nullptr, *d_total = nullptr;
@@ -243,6 +244,7 @@ struct jfc_graph {
     dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_defi); dfree(d_jf0); dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
     dfree(d_hmat);
+    dfree(d_touched);
     dfree(d_lines);
   }
   void release() {
@@ -309,7 +311,7 @@ struct jfc_graph {
       I.ty = (I.ny + th - 1) / th;
       I.tz = (I.nz + td - 1) / td;
       const long long nt = (long long)I.tx * I.ty * I.tz;
-      if (off + nt >= kEntryMat) return fail(JFC_ERR_INVALID, "graph too large for the tile queue");
+      if (off + nt >= kEntryTile) return fail(JFC_ERR_INVALID, "graph too large for the tile queue");
       I.tile_off = (int)off;
       I.ntiles = (int)nt;
       off += nt;
@@ -376,6 +378,7 @@ struct jfc_graph {
         (rc = dalloc(&d_order, ntiles, "tile order")) ||
         (rc = dalloc(&d_mat, ntiles, "tile states")) ||
         (rc = dalloc(&d_hmat, ntiles, "tile states")) ||
+        (rc = dalloc(&d_touched, ntiles, "tile states")) ||
         (rc = dalloc(&d_lines, 3 * n1, "tile lines")))
       return rc;
     JFC_HIP(hipMemcpy(d_lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -939,7 +942,8 @@ struct Solver {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
-      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, q, nullptr, total, nullptr, st);
+      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
+                     nullptr, st);
       flood(q, false);
     } else {
       flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles}, true);
@@ -956,8 +960,8 @@ struct Solver {
     auto issue_group = [&](int k) -> int {
       for (int j = 0; j < kPassGroup; ++j, ++pass) {
         Queue qc = queue_from_list();
-        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, qc, nullptr, total,
-                       nullptr, st);
+        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, nullptr, qc, nullptr,
+                       total, nullptr, st);
         flood(qc, false);
         advance();
         *passes += 1;
@@ -1037,11 +1041,11 @@ struct Solver {
     for (int c = 0; c < g->ncolors; ++c) {
       Queue q = queue_from_list();
       const int n = g->color_off[c + 1] - g->color_off[c];
-      launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, q,
-                     g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_);
+      launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, g->d_touched,
+                     q, g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_);
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
-                  g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,
+                  g->d_touched, g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,
                   g->d_total + kIterSlots, s_);
       if (timed && o.profile) push_ev.push_back({a, tm.mark()});
       advance();
@@ -1053,8 +1057,8 @@ struct Solver {
   // active tile in one list, Jacobi relabel, line pushes, active flags.
   void issue_fast_sweep() {
     Queue q = queue_from_list();
-    launch_compact(g->d_color_tiles, g->ntiles, 0, g->d_flag, nullptr, q, g->d_sweep_state,
-                   g->d_total, g->d_sweep_state + 2, st);
+    launch_compact(g->d_color_tiles, g->ntiles, 0, g->d_flag, nullptr, nullptr, q,
+                   g->d_sweep_state, g->d_total, g->d_sweep_state + 2, st);
     hipEvent_t a = o.profile ? tm.mark() : nullptr;
     launch_fc_sweep(g->shape, std::min(g->nwg, g->ntiles), g->tiles(), g->planes(), q,
                     g->d_flag, g->d_defi, st);
@@ -1125,6 +1129,7 @@ struct Solver {
       f.add(g->d_sweep_state, kSweepState, 0, 4);
       f.add(g->d_mat, g->ntiles, 0, 1);
       f.add(g->d_hmat, g->ntiles, 0, 1);
+      f.add(g->d_touched, g->ntiles, 0, 1);
       launch_fill_multi(f, st);
     }
     g->slot = 0;

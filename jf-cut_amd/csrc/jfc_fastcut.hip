@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_fc_relabel(Tiles T, Planes P, Queue q, 
   const int count = list_count(q);
   const Lane<S> L(threadIdx.x);
   for (int i = blockIdx.x; i < count; i += gridDim.x) {
-    const int t = q.list[i] & ~kEntryMat;
+    const int t = q.list[i] & kEntryTile;
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_fc_push(Tiles T, Planes P, Queue q, int
   const int slot = threadIdx.x / NLINES, line = threadIdx.x % NLINES;
   const long long st = P.stride;
   for (int i = blockIdx.x * PER_BLOCK + slot; i < count; i += gridDim.x * PER_BLOCK) {
-    const int t = q.list[i] & ~kEntryMat;
+    const int t = q.list[i] & kEntryTile;
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int tc = AXIS == 0 ? r.ltx : (AXIS == 1 ? r.lty : r.ltz);

@@ -172,13 +172,16 @@ constexpr int kSweepState = 8;
 void launch_sweep_end(const SweepCtl &c, hipStream_t st);
 void launch_sweep_resume(const SweepCtl &c, hipStream_t st);
 
-// Queue list entries: tile index, plus kEntryMat when compacted with `mat`
-// and mat[t] is set.
+// Queue list entries: tile index, plus, when compacted with `mat`, kEntryMat
+// if mat[t] is set, else kEntryClean if touched[t] is clear (no visit and no
+// flush has written the tile's planes or mask words in this solve).
 constexpr int kEntryMat = 1 << 30;
+constexpr int kEntryClean = 1 << 29;
+constexpr int kEntryTile = kEntryClean - 1;
 // stop (or null): a nonzero word makes the compaction a no-op.
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
-                    const unsigned char *mat, const Queue &q, int *sweep_cnt,
-                    long long *total, const int *stop, hipStream_t st);
+                    const unsigned char *mat, const unsigned char *touched, const Queue &q,
+                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st);
 // mat: per tile, 1 once every residual plane of the tile was written in
 // this solve (cleared when a solve starts).  defi: a visit rewrites its
 // tile's deficit flag (pushes never create deficits, so unvisited tiles'
@@ -188,10 +191,12 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // starts from.  max_iters caps the local iterations of a visit, sparse_iters
 // those of a launch whose list fits its grid.  iter_total: 64 int64 counters
 // of local iterations (or null).
+// touched: per tile, 1 once a flush wrote into the tile's planes or mask
+// words (cleared when a solve starts).
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
-                 long long *iter_total, hipStream_t st);
+                 unsigned char *touched, unsigned char *defi, unsigned char *hmat,
+                 int max_iters, int sparse_iters, long long *iter_total, hipStream_t st);
 // Fast-Cut comparator (jfc_fastcut.hip)
 void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);
 void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,

@@ -1003,7 +1003,8 @@ constexpr int kCompactBlock = 1024;
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n, int reverse,
                                                            unsigned char *flag,
-                                                           const unsigned char *mat, Queue q,
+                                                           const unsigned char *mat,
+                                                           const unsigned char *touched, Queue q,
                                                            int *sweep_cnt, long long *total,
                                                            const int *stop) {
   __shared__ int s_cnt[kCompactBlock / 64];
@@ -1039,7 +1040,9 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
   if (f) {
     int base = s_base;
     for (int k = 0; k < w; ++k) base += s_cnt[k];
-    q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = t | (mat && mat[t] ? kEntryMat : 0);
+    int ent = t;
+    if (mat) ent |= mat[t] ? kEntryMat : (touched && !touched[t] ? kEntryClean : 0);
+    q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = ent;
   }
 }
 
@@ -1095,11 +1098,11 @@ void launch_sweep_resume(const SweepCtl &c, hipStream_t st) {
 }
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
-                    const unsigned char *mat, const Queue &q, int *sweep_cnt,
-                    long long *total, const int *stop, hipStream_t st) {
+                    const unsigned char *mat, const unsigned char *touched, const Queue &q,
+                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st) {
   if (n <= 0) return;
   k_compact<<<(n + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, st>>>(
-      src, n, reverse, flag, mat, q, sweep_cnt, total, stop);
+      src, n, reverse, flag, mat, touched, q, sweep_cnt, total, stop);
 }
 
 // ---- persistent tile queue ----------------------------------------------------------
@@ -1297,9 +1300,9 @@ template <class S>
 // 8, where it spills).
 __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
-                                              unsigned char *defi, unsigned char *hmat,
-                                              int max_iters, int sparse_iters,
-                                              long long *iter_total) {
+                                              unsigned char *touched, unsigned char *defi,
+                                              unsigned char *hmat, int max_iters,
+                                              int sparse_iters, long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -1315,7 +1318,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
   int idx = -1, end = 0;
   while (Qc.next(idx, end)) {
     const int ent = q.list[idx];
-    const int t = ent & ~kEntryMat;
+    const int t = ent & kEntryTile;
     const TileRef r = tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
@@ -1327,9 +1330,13 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     const long long nn = (long long)I.nx * I.ny * I.nz;
     const int hcap = nn < HINF ? (int)nn : HINF;
     // A tile whose every plane is already written in this solve (kEntryMat,
-    // from mat[t]) loads cap directly; otherwise the per-plane fresh bits
-    // pick c0 or cap.
+    // from mat[t]) loads cap directly; a tile no visit or flush has written
+    // yet (kEntryClean) loads c0 directly, and its mask words are those of
+    // the inputs (every plane fresh, residual bits c0 > 0), so neither needs
+    // its mask word before its plane loads; otherwise the per-plane fresh
+    // bits pick c0 or cap.
     const bool fresh_tile = (ent & kEntryMat) == 0;
+    const bool clean = (ent & kEntryClean) != 0;
 
     // ---- loads: every global load of the visit is issued before any is
     // used; lanes off the grid, planes d >= K and dead halo cells read a
@@ -1374,6 +1381,12 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     if (!fresh_tile) {
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) c0r[d] = P.cap[ca[d]];
+    } else if (clean) {
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) c0r[d] = P.c0[ca[d]];
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d)
+        if (d < K) rm |= (1u << (8 + d)) | ((unsigned)(c0r[d] > 0) << d);
     } else {
       rm = P.rmask[gv];
 #pragma unroll
@@ -1504,9 +1517,13 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       const long long g = hc[c].g;
       const int f = s_e[p];
       if (f != 0) {
+        // flow entered g, so some plane of g changes below: g's tile is no
+        // longer clean
         atomicAdd(&P.e[g], f);
-        flag[tile_of<S>(I, r.ox + (p % S::PW) - 1, r.oy + (p / S::PW) % S::PH - 1,
-                        S::D3 ? r.oz + p / (S::PW * S::PH) - 1 : 0)] = 1;
+        const int ht = tile_of<S>(I, r.ox + (p % S::PW) - 1, r.oy + (p / S::PW) % S::PH - 1,
+                                  S::D3 ? r.oz + p / (S::PW * S::PH) - 1 : 0);
+        flag[ht] = 1;
+        touched[ht] = 1;
       }
       int dc[S::KMAX];
       unsigned fresh = 0, flip = 0;
@@ -1548,10 +1565,11 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
 
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
-                 unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
-                 long long *iter_total, hipStream_t st) {
+                 unsigned char *touched, unsigned char *defi, unsigned char *hmat,
+                 int max_iters, int sparse_iters, long long *iter_total, hipStream_t st) {
   JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
-                          T, P, q, flag, mat, defi, hmat, max_iters, sparse_iters, iter_total)));
+                          T, P, q, flag, mat, touched, defi, hmat, max_iters, sparse_iters,
+                          iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
