@@ -337,6 +337,23 @@ int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
                     int bytes_per_channel, const uint8_t *seeds, int on_device,
                     const jfc_model_params *params, jfc_model_stats *stats);
 
+/* jfc_model_image without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  image and
+ * seeds must be device pointers (there is no staging buffer).  The scratch is
+ * a static device array of JFC_MODEL_STATIC_BYTES bytes per GPU, held until
+ * the process ends; it limits the seeded nodes (both sides together) to
+ * JFC_MODEL_STATIC_MAX_SEEDS, beyond which the call returns JFC_ERR_INVALID
+ * before writing any plane.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (from several threads or streams) race and
+ * must be serialised by the caller.  Results are identical to
+ * jfc_model_image's. */
+#define JFC_MODEL_STATIC_MAX_SEEDS (1 << 22)
+#define JFC_MODEL_STATIC_BYTES (136448 + 4 * JFC_MODEL_STATIC_MAX_SEEDS)
+int jfc_model_image_static(jfc_graph *g, int inst, const void *image, int channels,
+                           int bytes_per_channel, const uint8_t *seeds,
+                           const jfc_model_params *params, jfc_model_stats *stats);
+
 /* ---- the widget's own data terms (QGCW:2560-2606 with its "Modeling" and
  * "Feature" panels, QGCPanel.ui) ------------------------------------------- */
 

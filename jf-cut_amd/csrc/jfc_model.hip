@@ -21,13 +21,16 @@
 // The whole model runs on the caller's stream with three host waits (the
 // seed counts, the pair sums for the thresholds, the end); the k-means
 // iterations run on the device (each update raises a done flag that turns
-// the later launches into no-ops).  The scratch is one small block the graph
-// allocates on its first model call, plus a seed list.
+// the later launches into no-ops).  The scratch is one small block plus a
+// seed list: the graph's own (allocated on its first model call, the list
+// borrowing the excess plane) for jfc_model_image, or a static device array
+// for jfc_model_image_static, which allocates nothing.
 #include "jfcut.h"
 #include "jfc_device.h"
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 namespace jfc {
@@ -671,4 +674,58 @@ extern "C" int jfc_model_image(jfc_graph *g, int inst, const void *image, int ch
   long long cap = 0;
   if ((rc = graph_work_plane(g, inst, &list, &cap))) return rc;
   return model_common(g, inst, img, channels, bpc, sd, prm, stats, scratch, list, cap);
+}
+// ---- the entry point that allocates nothing --------------------------------------
+//
+// jfc_model_image_static keeps its scratch in this file-scope device array:
+// each GPU holds one copy for the life of the process, at an address that
+// hipGetSymbolAddress returns for the device that is current.  Host code
+// never passes the array's name to a kernel (in host code it is a stand-in,
+// not a device address); it passes the per-device address instead.
+constexpr long long kStaticSeeds = JFC_MODEL_STATIC_MAX_SEEDS;  // both sides together
+constexpr size_t kStaticBytes = kSmallBytes + (size_t)kStaticSeeds * 4;
+static_assert(kStaticBytes == (size_t)JFC_MODEL_STATIC_BYTES, "jfcut.h states the array's size");
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// Address of the current device's copy of `scratch`, looked up once per
+// device with that device current.
+int static_scratch(void **p) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return model_fail(JFC_ERR_DEVICE, "no current HIP device for the static scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return model_fail(JFC_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *p = g_scratch_ptr[dev];
+  return JFC_OK;
+}
+}  // namespace
+
+extern "C" int jfc_model_image_static(jfc_graph *g, int inst, const void *image, int channels,
+                                      int bpc, const uint8_t *seeds,
+                                      const jfc_model_params *prm, jfc_model_stats *stats) {
+  int rc = check_model_params(channels, bpc, prm);
+  if (rc) return rc;
+  InstDesc I;
+  int *cs, *ct, *c0;
+  long long stride;
+  if ((rc = graph_instance(g, inst, &I, &cs, &ct, &c0, &stride))) return rc;
+  if (!image || !seeds) return model_fail(JFC_ERR_INVALID, "null argument");
+  const long long n = (long long)I.nx * I.ny * I.nz;
+  if (n >= (1LL << 31)) return model_fail(JFC_ERR_INVALID, "instance too large for the model path");
+  void *base = nullptr;
+  if ((rc = static_scratch(&base))) return rc;
+  return model_common(g, inst, image, channels, bpc, seeds, prm, stats, base,
+                      (int *)((unsigned char *)base + kSmallBytes), kStaticSeeds);
 }

@@ -102,7 +102,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
            "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_get_initial_relabel", "jfc_default_widget_model_params",
-           "jfc_model_widget", "jfc_set_stream", "jfc_input_planes", "jfc_inputs_written"]
+           "jfc_model_widget", "jfc_set_stream", "jfc_input_planes", "jfc_inputs_written",
+           "jfc_model_image_static"]
 
 
 def lib():
@@ -137,6 +138,8 @@ def lib():
         "jfc_backward": (I, [P, I, P, P, P, I]),
         "jfc_model_image": (I, [P, I, P, I, I, P, I, ctypes.POINTER(ModelParams),
                                 ctypes.POINTER(ModelStats)]),
+        "jfc_model_image_static": (I, [P, I, P, I, I, P, ctypes.POINTER(ModelParams),
+                                       ctypes.POINTER(ModelStats)]),
         "jfc_synth_image": (I, [P, P, I, I, I, ctypes.c_double, ctypes.c_uint64]),
         "jfc_synth_video": (I, [P, P, I, I, I, I, ctypes.c_double, ctypes.c_uint64]),
         "jfc_synth_volume": (I, [P, P, I, I, I, I, ctypes.c_uint64]),
