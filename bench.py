@@ -57,6 +57,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def data_term_ms(wl, reps=3):
+    """The data term of the workload (integer k-means per side, unary and
+    pairwise capacities, jfc_model_image) with the image and seeds resident
+    in HBM: wall ms per call, the call's host waits included."""
+    W, H = wl["W"], wl["H"]
+    rgb, sd = jfcut.synth_image(W, H, nblobs=wl["nblobs"], noise=wl["noise"], seed=0)
+    L = jfcut.lib()
+    hip = ctypes.CDLL("libamdhip64.so")
+    bi, bs = DeviceBuffer(rgb.nbytes), DeviceBuffer(sd.nbytes)
+    hip.hipMemcpy(ctypes.c_void_p(bi.data_ptr()), rgb.ctypes.data_as(ctypes.c_void_p),
+                  ctypes.c_size_t(rgb.nbytes), 1)
+    hip.hipMemcpy(ctypes.c_void_p(bs.data_ptr()), sd.ctypes.data_as(ctypes.c_void_p),
+                  ctypes.c_size_t(sd.nbytes), 1)
+    g = jfcut.GridGraph.grid(W, H, 1, wl["conn"])
+    prm = jfcut.ModelParams(wl["k"], wl["k"], 32, wl["unary"], 255, float(wl["lam"]),
+                            float(wl["lam"]), 0)
+    ms = []
+    for i in range(reps + 1):
+        hip_sync()
+        t0 = time.perf_counter()
+        jfcut._check(L.jfc_model_image(g._h, 0, ctypes.c_void_p(bi.data_ptr()), 3, 1,
+                                       ctypes.c_void_p(bs.data_ptr()), 1, ctypes.byref(prm), None))
+        hip_sync()
+        if i:
+            ms.append((time.perf_counter() - t0) * 1e3)
+    g.close()
+    bi.free()
+    bs.free()
+    return round(float(np.median(ms)), 3)
+
+
 def build_graph(wl, seed=0, W=None, H=None):
     W = W or wl["W"]
     H = H or wl["H"]
@@ -262,10 +293,13 @@ def fixtures_leg(reps=5):
     """The reference's own graphs (flower 600x450, person 600x450, sponge
     640x480, 4-connected; tests/golden/ref_*.npz hold the reference's
     .ef/.c1/.c2 arrays): GPU ms per fresh solve and BK303 ms on the same
-    graphs in the same run, flows equal to the published values."""
+    graphs in the same run, flows equal to the published values.  The three
+    graphs share one size class, autotuned once on flower (outside the
+    timing) as a caller would for its image size."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     out = {"published_jfcut_flower_ms_gtx_titan": FLOWER_TITAN_MS}
+    tuned = None
     for name, flow in FIXTURE_FLOWS.items():
         z = np.load(os.path.join(ROOT, "tests", "golden", f"ref_{name}.npz"))
         nx, ny = int(z["nx"]), int(z["ny"])
@@ -275,12 +309,17 @@ def fixtures_leg(reps=5):
         g.set_excess(ef)
         for d in range(4):
             g.set_neighbor_caps(d, caps[d])
+        if tuned is None:
+            g.autotune(reps=2)
+            tuned = g.tuning()
+            out["tuning"] = tuned
         g.solve()
         ms, stats = timed_steps(g, reps, fresh=True)
         assert g.flow() == flow, (name, g.flow(), flow)
         dev = float(np.median([s.ms_total for s in stats]))
         g.close()
         rec = {"nodes": nx * ny, "gpu_ms": round(ms, 3), "gpu_device_ms": round(dev, 3),
+               "sweeps": stats[-1].sweeps, "global_relabels": stats[-1].global_relabels,
                "flow": flow, "flow_equal_published": True}
         if ol.have_bk():
             cs, ct = np.maximum(ef, 0), np.maximum(-ef, 0)
@@ -303,7 +342,7 @@ def config_leg(name, steps=3):
     del img, sd
     c = jfcut.CONFIGS[name]
     n = c["W"] * c["H"] * c.get("T", c.get("D", 1))
-    tile = g.autotune(reps=1)
+    g.autotune(reps=1)
     ms, stats = timed_steps(g, steps, fresh=True)
     flows = {s.flow for s in stats}
     assert len(flows) == 1 and all(s.converged == 1 for s in stats)
@@ -311,8 +350,7 @@ def config_leg(name, steps=3):
     st = stats[-1]
     out = {"nodes": n, "ms_per_solve": round(ms, 3),
            "mvoxels_per_s": round(n / (ms * 1e-3) / 1e6, 1),
-           "flow": int(stats[0].flow), "tile_shape": tile, "local_iters": g.local_iters(),
-           "initial_relabel": g.initial_relabel(), "sweeps": st.sweeps,
+           "flow": int(stats[0].flow), "tuning": g.tuning(), "sweeps": st.sweeps,
            "global_relabels": st.global_relabels, "relabel_passes": st.relabel_passes,
            "ms_relabel": round(st.ms_relabel, 3),
            "whole_solve_frac": round(st.solve_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
@@ -404,11 +442,10 @@ def main():
     t0 = time.time()
     g = build_graph(wl, seed=0)
     log(f"workload built in {time.time() - t0:.1f}s")
-    tile = None
     if not args.no_autotune:
-        tile = g.autotune(reps=2)
-        log(f"autotuned: tile {tile}, local iterations {g.local_iters()}, "
-            f"initial relabel {g.initial_relabel()}")
+        g.autotune(reps=2)
+    tuning = g.tuning()
+    log(f"tuning: {tuning}")
     for i in range(args.warmup):
         g.inputs_written()
         st = g.solve()
@@ -473,9 +510,12 @@ def main():
         fixtures = fixtures_leg()
         log(f"fixtures: {fixtures}")
 
+    dterm = data_term_ms(wl)
+    log(f"data term: {dterm} ms")
+
     seeds_only = None
     if not args.no_seeds_only and args.workload == "c2":
-        seeds_only = seeds_only_leg(wl["W"], wl["H"])
+        seeds_only = seeds_only_leg(wl["W"] // 4, wl["H"] // 4)
         log(f"seeds only: {seeds_only}")
 
     volumes = None
@@ -513,7 +553,8 @@ def main():
                    "ms_no_events": round(plain_ms, 3), "ms_warm_resolve": round(warm_ms, 3),
                    "ms_with_device_setters": round(set_ms, 3),
                    "mvoxels_per_s_warm": round(N / (warm_ms * 1e-3) / 1e6, 1),
-                   "tile_shape": tile, "sweeps": stats[-1].sweeps,
+                   "data_term_ms": dterm,
+                   "tuning": tuning, "sweeps": stats[-1].sweeps,
                    "global_relabels": stats[-1].global_relabels,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -107,9 +107,15 @@ typedef struct {
   int sparse_local_iters; /* cap on LDS discharge iterations per visit in a
                              push launch whose active-tile list is no longer
                              than its grid (each workgroup visits at most one
-                             tile): 0 = 64 (default), negative = max_local_iters
-                             (the reference loops each visit until no push,
-                             graphcut_push_relabel.cl:77-182) */
+                             tile): 0 (default) = the cap jfc_autotune chose
+                             for the graph's size class, else 32; negative =
+                             max_local_iters (the reference loops each visit
+                             until no push, graphcut_push_relabel.cl:77-182) */
+  int stall_certify;      /* when the active set stalls (stall_ratio): 1 = run
+                             the canonical-cut search first and finish if it
+                             finds no augmenting path, else relabel; 0 =
+                             relabel; -1 (default) = the choice jfc_autotune
+                             made for the graph's size class, else 0 */
 } jfc_options;
 
 #define JFC_METHOD_JFCUT 0
@@ -287,10 +293,13 @@ int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
 /* Time `reps` solves of the graph (after one warm-up) with every tile shape of
  * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2), crossed with
  * every local-iteration cap of {2, 3, 4} unless opt->max_local_iters fixes
- * it and with initial_relabel 0 and 1 unless opt->initial_relabel fixes it,
- * under the other options of opt (may be NULL), and keep the fastest choice
- * as what the graph's solves use when opt->tile_shape is JFC_TILE_DEFAULT,
- * opt->max_local_iters is 0 and opt->initial_relabel is -1.  The choice is also stored for the graph's size
+ * it and with initial_relabel 0 and 1 unless opt->initial_relabel fixes it;
+ * then, on the fastest of those, stall_certify 1 unless opt->stall_certify
+ * fixes it, and then sparse_local_iters -1, 8 and 64 unless
+ * opt->sparse_local_iters fixes it; all under the other options of opt (may
+ * be NULL).  The fastest choice is what the graph's solves use when those
+ * options are left at their defaults (JFC_TILE_DEFAULT, 0, -1, -1, 0).  The
+ * choice is also stored for the graph's size
  * class (dimensionality, connectivities, instance count and mean instance
  * size, each rounded to a power of two): graphs of the class created later in
  * the process start with it.  Results are unchanged by either choice.
@@ -305,6 +314,12 @@ int jfc_get_local_iters(const jfc_graph *g, int *max_local_iters);
 
 /* The initial_relabel value (0 or 1) a solve with initial_relabel = -1 uses. */
 int jfc_get_initial_relabel(const jfc_graph *g, int *initial_relabel);
+
+/* Every tuned choice at once: writes tile_shape, max_local_iters,
+ * initial_relabel, stall_certify and sparse_local_iters of *opt with the
+ * values solves that leave them at their defaults use (other fields are
+ * not touched). */
+int jfc_get_tuning(const jfc_graph *g, jfc_options *opt);
 
 /* ---- data term (QGCW:2560-2606 slotModelClicked, QGCW:1997-2107 kmeans) --- */
 

@@ -92,7 +92,15 @@ struct Tuned {
   int shape = 0;
   int mli = 0;
   int ir = -1;  // initial relabel (-1: not tuned)
+  int sc = -1;  // certify first on a stall (-1: not tuned)
+  int sp = 0;   // cap of sparse launches (0: not tuned)
 };
+
+// Default cap on the local iterations of a launch whose list fits its grid
+// (the reference fixtures: 4.6 -> 2.2-3.0 ms against a cap of 2; larger caps
+// slow graphs whose few active tiles iterate without effect, config 1 0.48
+// -> 0.71 ms at 64).
+constexpr int kSparseIters = 32;
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tune;
 
@@ -118,6 +126,8 @@ struct jfc_graph {
   int tuned = 0;           // JFC_TILE_* chosen by jfc_autotune (0 = none)
   int tuned_mli = 0;       // local-iteration cap chosen by jfc_autotune (0 = none)
   int tuned_ir = -1;       // initial relabel chosen by jfc_autotune (-1 = none)
+  int tuned_sc = -1;       // stall_certify chosen by jfc_autotune (-1 = none)
+  int tuned_sp = 0;        // sparse_local_iters chosen by jfc_autotune (0 = none)
   std::string size_class;  // key of the process-wide autotuning table
   int Kg = 4;  // planes stored (max K over instances)
   int shape = 0;
@@ -280,7 +290,11 @@ struct jfc_graph {
   }
 
   Tiles tiles() const {
-    return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows};
+    // the whole-graph passes stream 4-node groups, 1024 per workgroup step:
+    // a one-instance graph with short rows takes several rows per step
+    int step = 1;
+    if (inst.size() == 1) step = std::max(1, 1024 / ((inst[0].nx + 3) / 4));
+    return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows, step};
   }
   Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot, rmask}; }
   TileLines lines() const {
@@ -471,6 +485,8 @@ extern "C" void jfc_default_options(jfc_options *o) {
   o->graph_sweeps = 1;
   o->tile_order = 0;
   o->method = JFC_METHOD_JFCUT;
+  o->sparse_local_iters = 0;
+  o->stall_certify = -1;
 }
 
 static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
@@ -577,6 +593,8 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       g->tuned = it->second.shape;
       g->tuned_mli = it->second.mli;
       g->tuned_ir = it->second.ir;
+      g->tuned_sc = it->second.sc;
+      g->tuned_sp = it->second.sp;
     }
   }
   if ((rc = g->set_shape(default_shape(g->dim)))) { delete g; return rc; }
@@ -1201,7 +1219,18 @@ struct Solver {
       }
       // the sweeps queued behind the stop were no-ops
       if (stopw == 3) break;  // sweep budget spent
-      if (stopw == 2) {
+      if (stopw == 2 && o.stall_certify > 0) {
+        // a stalled active set is often excess cut off from every deficit:
+        // the canonical-cut search proves that (and is the solve's last
+        // step) at less cost than the relabel that would show it
+        int found = 0;
+        if ((rc = canonical_cut(&found))) return rc;
+        if (!found) {
+          done = true;
+          break;
+        }
+      }
+      if (stopw == 2 || stopw == 4) {
         if ((rc = global_relabel())) return rc;
         launch_sweep_resume(sweep_ctl(), st);
         continue;
@@ -1287,11 +1316,12 @@ extern "C" int jfc_solve(jfc_graph *g, const jfc_options *opt, jfc_stats *stats)
   if (S.o.check_every <= 0) S.o.check_every = 1;
   // 0: the cap jfc_autotune chose for the graph's size class, else 2
   if (S.o.max_local_iters <= 0) S.o.max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
-  // 0: 64; negative: the dense cap
-  if (S.o.sparse_local_iters == 0) S.o.sparse_local_iters = 64;
+  // 0: the tuned cap, else kSparseIters; negative: the dense cap
+  if (S.o.sparse_local_iters == 0) S.o.sparse_local_iters = g->tuned_sp ? g->tuned_sp : kSparseIters;
   if (S.o.sparse_local_iters < 0) S.o.sparse_local_iters = S.o.max_local_iters;
   // -1: the choice jfc_autotune made for the size class, else none
   if (S.o.initial_relabel < 0) S.o.initial_relabel = g->tuned_ir > 0 ? 1 : 0;
+  if (S.o.stall_certify < 0) S.o.stall_certify = g->tuned_sc > 0 ? 1 : 0;
   if (S.o.tile_order != 0 && S.o.tile_order != 1)
     return fail(JFC_ERR_INVALID, "tile_order must be 0 (distance transform) or 1 (jump flooding)");
   if (S.o.method != JFC_METHOD_JFCUT && S.o.method != JFC_METHOD_FASTCUT)
@@ -1491,38 +1521,82 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   // the push sweeps settle most of the flow first (config 2: 4.8 -> 8.5 ms)
   const int ir_fixed = o.initial_relabel >= 0 ? o.initial_relabel : -1;
   const int ni = ir_fixed >= 0 ? 1 : 2;
-  int best = 0, best_mli = 0, best_ir = 0;
+  // then, one at a time on the best of those: whether a stalled active set
+  // first tries the certificate (it pays where the last relabel would only
+  // prove the remaining excess cut off), and the cap of sparse launches
+  // (large where flow must cross whole tiles, small where visits of a few
+  // tiles would iterate without effect)
+  const int sc_fixed = o.stall_certify >= 0 ? o.stall_certify : -1;
+  const int sp_fixed = o.sparse_local_iters != 0 ? o.sparse_local_iters : 0;
+  auto time_it = [&](const jfc_options &t, double *med) -> int {
+    std::vector<double> ms;
+    for (int r = 0; r <= reps; ++r) {
+      jfc_stats st;
+      const int rc = jfc_solve(g, &t, &st);
+      if (rc < 0) return rc;
+      if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
+    }
+    std::sort(ms.begin(), ms.end());
+    *med = ms[ms.size() / 2];
+    return JFC_OK;
+  };
+  jfc_options best_o = o;
   double best_ms = 0.0;
+  bool have = false;
+  auto consider = [&](const jfc_options &t) -> int {
+    double med = 0.0;
+    if (int rc = time_it(t, &med)) return rc;
+    if (!have || med < best_ms) {
+      best_o = t;
+      best_ms = med;
+      have = true;
+    }
+    return JFC_OK;
+  };
+  o.stall_certify = sc_fixed >= 0 ? sc_fixed : 0;
+  o.sparse_local_iters = sp_fixed ? sp_fixed : kSparseIters;
   for (int c = 0; c < nc; ++c)
     for (int m = 0; m < nm; ++m)
       for (int i = 0; i < ni; ++i) {
         o.tile_shape = cand[c];
         o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
         o.initial_relabel = ir_fixed >= 0 ? ir_fixed : i;
-        std::vector<double> ms;
-        for (int r = 0; r <= reps; ++r) {
-          jfc_stats st;
-          const int rc = jfc_solve(g, &o, &st);
-          if (rc < 0) return rc;
-          if (r > 0) ms.push_back(st.ms_total);  // the first run warms the shape's tables
-        }
-        std::sort(ms.begin(), ms.end());
-        const double med = ms[ms.size() / 2];
-        if (!best || med < best_ms) {
-          best = cand[c];
-          best_mli = o.max_local_iters;
-          best_ir = o.initial_relabel;
-          best_ms = med;
-        }
+        if (int rc = consider(o)) return rc;
       }
-  g->tuned = best;
-  g->tuned_mli = mli_fixed ? 0 : best_mli;
-  g->tuned_ir = ir_fixed >= 0 ? -1 : best_ir;
+  if (sc_fixed < 0) {
+    jfc_options t = best_o;
+    t.stall_certify = 1;
+    if (int rc = consider(t)) return rc;
+  }
+  if (!sp_fixed) {
+    const int spc[3] = {-1, 8, 64};
+    const jfc_options base = best_o;
+    for (int v : spc) {
+      jfc_options t = base;
+      t.sparse_local_iters = v;
+      if (int rc = consider(t)) return rc;
+    }
+  }
+  g->tuned = best_o.tile_shape;
+  g->tuned_mli = mli_fixed ? 0 : best_o.max_local_iters;
+  g->tuned_ir = ir_fixed >= 0 ? -1 : best_o.initial_relabel;
+  g->tuned_sc = sc_fixed >= 0 ? -1 : best_o.stall_certify;
+  g->tuned_sp = sp_fixed ? 0 : best_o.sparse_local_iters;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[g->size_class] = Tuned{best, g->tuned_mli, g->tuned_ir};
+    g_tune[g->size_class] = Tuned{g->tuned, g->tuned_mli, g->tuned_ir, g->tuned_sc, g->tuned_sp};
   }
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = g->tuned;
+  return JFC_OK;
+}
+
+extern "C" int jfc_get_tuning(const jfc_graph *g, jfc_options *opt) {
+  if (!g || !opt) return fail(JFC_ERR_INVALID, "null argument");
+  opt->tile_shape = g->tuned ? g->tuned : (g->dim == 3 ? JFC_TILE_8x8x4 : JFC_TILE_32x8);
+  opt->max_local_iters = g->tuned_mli > 0 ? g->tuned_mli : 2;
+  opt->initial_relabel = g->tuned_ir > 0 ? 1 : 0;
+  opt->stall_certify = g->tuned_sc > 0 ? 1 : 0;
+  opt->sparse_local_iters = g->tuned_sp ? g->tuned_sp : kSparseIters;
   return JFC_OK;
 }
 

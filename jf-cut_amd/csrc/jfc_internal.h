@@ -67,6 +67,8 @@ struct Tiles {
   const int *row_off;    // first grid row (y, z) of every instance, + total
   int ninst;
   int nrows;             // rows of nx nodes over all instances
+  int row_step;          // rows a workgroup streams at once in the whole-graph
+                         // passes (several short rows of a one-instance graph)
 };
 
 // Tile shapes (256 nodes each, one node per lane).
@@ -157,7 +159,8 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
 
 // Sweep bookkeeping on the device.  state (8 ints): [0] active tiles of the
 // running sweep, [1] sweep index, [2] stop word (0 sweep on, 1 no active
-// tile, 2 relabel due, 3 sweep budget spent), [3] sweep index of the last
+// tile, 2 active set stalled, 3 sweep budget spent, 4 relabel budget
+// spent), [3] sweep index of the last
 // relabel, [4..5] tile visits (int64) at the last relabel.
 struct SweepCtl {
   int *hist;             // active tiles per sweep

@@ -297,54 +297,6 @@ __device__ __forceinline__ long long div_floor(long long x, long long den) {
   return q;
 }
 
-// cs = round(scale * d2_bg / den), ct = round(scale * d2_fg / den); seeds get
-// (2^30, 0) / (0, 2^30).  Rows of the image (unpadded, nx nodes) go to rows
-// of the planes (pitch px); lanes walk a row, workgroups walk rows.
-template <int BPC>
-__global__ __launch_bounds__(256) void k_unary(const void *img, int C, const unsigned char *seeds,
-                                               int nx, long long rows, int px, Scratch s,
-                                               int Kf, int Kb, long long den, long long scale,
-                                               int *cs, int *ct) {
-  __shared__ long long s_f[KMAX_CLUSTERS * 3], s_b[KMAX_CLUSTERS * 3];
-  for (int i = threadIdx.x; i < Kf * C; i += blockDim.x) s_f[i] = s.cent[i];
-  for (int i = threadIdx.x; i < Kb * C; i += blockDim.x) s_b[i] = s.cent[KMAX_CLUSTERS * 3 + i];
-  __syncthreads();
-  const long long half = den / 2;
-  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
-    const long long v0 = r * nx, o0 = r * px;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-      const long long v = v0 + x;
-      const int sd = seeds[v];
-      int a, b;
-      if (sd == 1) { a = 1 << 30; b = 0; }
-      else if (sd == 2) { a = 0; b = 1 << 30; }
-      else {
-        int f[3] = {0, 0, 0};
-        for (int c = 0; c < C; ++c) f[c] = pixel<BPC>(img, v * C + c);
-        long long df = km_d2<BPC>(f, s_f, C), db = km_d2<BPC>(f, s_b, C);
-        for (int k = 1; k < Kf; ++k) df = min(df, km_d2<BPC>(f, s_f + k * C, C));
-        for (int k = 1; k < Kb; ++k) db = min(db, km_d2<BPC>(f, s_b + k * C, C));
-        a = (int)div_floor(scale * db + half, den);
-        b = (int)div_floor(scale * df + half, den);
-      }
-      cs[o0 + x] = a;
-      ct[o0 + x] = b;
-    }
-  }
-}
-
-// Squared feature difference of two nodes.
-template <int BPC>
-__device__ __forceinline__ unsigned long long sqdiff(const void *img, int C, long long a,
-                                                     long long b) {
-  unsigned long long D = 0;
-  for (int c = 0; c < C; ++c) {
-    const long long t = pixel<BPC>(img, a * C + c) - pixel<BPC>(img, b * C + c);
-    D += (unsigned long long)(t * t);
-  }
-  return D;
-}
-
 // The undirected edges a node owns: the forward directions (even d) whose
 // neighbour lies inside the grid; each edge is owned by exactly one node.
 // 2D: +x, +y and, 8-connected, (+x,+y), (-x,+y); 3D: +x, +y, +z.
@@ -385,6 +337,150 @@ struct Fwd {
   }
 };
 
+// ---- row staging -----------------------------------------------------------------
+//
+// The per-node data-term kernels work on segments of SEG pixels of one image
+// row (lane l takes pixels x0 + l + 256 j, so every global store of a wave is
+// 64 consecutive words), with the image bytes of the rows a segment reads
+// staged in LDS through aligned 4-byte loads: a pixel is 1 to 3 bytes, and
+// per-channel byte loads from HBM would spend one memory instruction per
+// byte per lane.
+constexpr int SEG = 1024;                          // pixels per work item
+constexpr int ROWW = (3 * (SEG + 2) + 3) / 4 + 2;  // words of one staged row
+
+struct RowStage {
+  int off;  // byte of the first staged pixel inside the staged words
+};
+
+// Stage pixels [a, b) of image row `row` (nx pixels per row, bpp bytes per
+// pixel, `total` bytes in the image) into `words`.  Words that end past the
+// image are assembled from single bytes, so no load leaves the buffer.
+__device__ __forceinline__ RowStage stage_row(const unsigned char *img, long long total,
+                                              long long row, int nx, int bpp, int a, int b,
+                                              unsigned *words) {
+  const long long first = (row * nx + a) * bpp, last = (row * nx + b) * bpp;
+  const long long w0 = first >> 2, w1 = (last + 3) >> 2;
+  for (long long w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
+    unsigned v;
+    if ((w << 2) + 4 <= total) {
+      v = ((const unsigned *)img)[w];
+    } else {
+      v = 0;
+      for (int k = 0; k < 4; ++k)
+        if ((w << 2) + k < total) v |= (unsigned)img[(w << 2) + k] << (8 * k);
+    }
+    words[w - w0] = v;
+  }
+  RowStage s;
+  s.off = (int)(first - (w0 << 2));
+  return s;
+}
+
+// Channel c of staged pixel p (p counted from the stage's first pixel).
+template <int BPC>
+__device__ __forceinline__ int staged(const unsigned *words, const RowStage &s, int p, int bpp,
+                                      int c) {
+  const unsigned char *b = (const unsigned char *)words + s.off + p * bpp;
+  return BPC == 1 ? (int)b[c] : (int)((const unsigned short *)b)[c];
+}
+
+template <int BPC>
+__device__ __forceinline__ unsigned long long staged_sqdiff(const unsigned *wa,
+                                                            const RowStage &sa, int pa,
+                                                            const unsigned *wb,
+                                                            const RowStage &sb, int pb,
+                                                            int bpp, int C) {
+  unsigned long long D = 0;
+  for (int c = 0; c < C; ++c) {
+    const long long t = staged<BPC>(wa, sa, pa, bpp, c) - staged<BPC>(wb, sb, pb, bpp, c);
+    D += (unsigned long long)(t * t);
+  }
+  return D;
+}
+
+// cs = round(scale * d2_bg / den), ct = round(scale * d2_fg / den); seeds get
+// (2^30, 0) / (0, 2^30).  Rows of the image (unpadded, nx nodes) go to rows
+// of the planes (pitch px).
+template <int BPC>
+__global__ __launch_bounds__(256) void k_unary(const void *img, int C, const unsigned char *seeds,
+                                               int nx, long long rows, int px, Scratch s,
+                                               int Kf, int Kb, long long den, long long scale,
+                                               int *cs, int *ct) {
+  __shared__ long long s_f[KMAX_CLUSTERS * 3], s_b[KMAX_CLUSTERS * 3];
+  __shared__ unsigned s_row[ROWW];
+  for (int i = threadIdx.x; i < Kf * C; i += blockDim.x) s_f[i] = s.cent[i];
+  for (int i = threadIdx.x; i < Kb * C; i += blockDim.x) s_b[i] = s.cent[KMAX_CLUSTERS * 3 + i];
+  const int bpp = C * BPC;
+  const long long total = rows * nx * bpp;
+  const long long half = den / 2;
+  const int segs = (nx + SEG - 1) / SEG;
+  for (long long item = blockIdx.x; item < rows * segs; item += gridDim.x) {
+    const long long r = item / segs;
+    const int x0 = (int)(item % segs) * SEG, x1 = min(nx, x0 + SEG);
+    __syncthreads();  // the previous item's reads of s_row are done
+    const RowStage rs = stage_row((const unsigned char *)img, total, r, nx, bpp, x0, x1, s_row);
+    __syncthreads();
+    for (int x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
+      const long long v = r * nx + x;
+      const int sd = seeds[v];
+      int a, b;
+      if (sd == 1) { a = 1 << 30; b = 0; }
+      else if (sd == 2) { a = 0; b = 1 << 30; }
+      else {
+        int f[3] = {0, 0, 0};
+        for (int c = 0; c < C; ++c) f[c] = staged<BPC>(s_row, rs, x - x0, bpp, c);
+        long long df = km_d2<BPC>(f, s_f, C), db = km_d2<BPC>(f, s_b, C);
+        for (int k = 1; k < Kf; ++k) df = min(df, km_d2<BPC>(f, s_f + k * C, C));
+        for (int k = 1; k < Kb; ++k) db = min(db, km_d2<BPC>(f, s_b + k * C, C));
+        a = (int)div_floor(scale * db + half, den);
+        b = (int)div_floor(scale * df + half, den);
+      }
+      cs[r * px + x] = a;
+      ct[r * px + x] = b;
+    }
+  }
+}
+
+// The rows a segment of row r = y + ny z reads for its forward edges: its own
+// row (x0 .. x1, plus the pixel right of it), the next row of the slice
+// (x0 - 1 .. x1 + 1) and, in 3D, the same row of the next slice.
+struct Stages {
+  RowStage own, next, up;
+  int a_own, a_next, a_up;  // first staged pixel of each
+};
+
+template <int CONN>
+__device__ __forceinline__ Stages stage_rows(const unsigned char *img, long long total, int nx,
+                                             int ny, int nz, int bpp, long long r, int x0,
+                                             int x1, unsigned *w_own, unsigned *w_next,
+                                             unsigned *w_up) {
+  const int y = (int)(r % ny), z = (int)(r / ny);
+  Stages S;
+  S.a_own = x0;
+  S.own = stage_row(img, total, r, nx, bpp, x0, min(nx, x1 + 1), w_own);
+  S.a_next = max(0, x0 - 1);
+  if (y + 1 < ny) S.next = stage_row(img, total, r + 1, nx, bpp, S.a_next, min(nx, x1 + 1), w_next);
+  S.a_up = x0;
+  if (CONN == 6 && z + 1 < nz) S.up = stage_row(img, total, r + ny, nx, bpp, x0, x1, w_up);
+  return S;
+}
+
+// Squared difference of forward edge j of node (x, y, z), all from LDS.
+template <int CONN, int BPC>
+__device__ __forceinline__ unsigned long long edge_d(const Stages &S, const unsigned *w_own,
+                                                     const unsigned *w_next, const unsigned *w_up,
+                                                     int x, int j, int bpp, int C) {
+  const int p = x - S.a_own;
+  switch (2 * j) {
+    case 0: return staged_sqdiff<BPC>(w_own, S.own, p, w_own, S.own, p + 1, bpp, C);
+    case 2: return staged_sqdiff<BPC>(w_own, S.own, p, w_next, S.next, x - S.a_next, bpp, C);
+    case 4:
+      if (CONN == 6) return staged_sqdiff<BPC>(w_own, S.own, p, w_up, S.up, x - S.a_up, bpp, C);
+      return staged_sqdiff<BPC>(w_own, S.own, p, w_next, S.next, x + 1 - S.a_next, bpp, C);
+    default: return staged_sqdiff<BPC>(w_own, S.own, p, w_next, S.next, x - 1 - S.a_next, bpp, C);
+  }
+}
+
 // Sum of squared neighbour differences per edge group (0: x/y and the 2D
 // diagonals, and z unless it forms its own group; 1: z when separate), one
 // term per undirected edge.
@@ -392,17 +488,25 @@ template <int CONN, int BPC>
 __global__ __launch_bounds__(256) void k_pair_sums(const void *img, int C, int nx, int ny,
                                                    int nz, int zsep, long long *sums) {
   __shared__ long long buf[4][4];
+  __shared__ unsigned w_own[ROWW], w_next[ROWW], w_up[CONN == 6 ? ROWW : 1];
   unsigned long long s0 = 0, s1 = 0;
   long long p0 = 0, p1 = 0;
-  const long long rows = (long long)ny * nz, plane = (long long)nx * ny;
-  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+  const int bpp = C * BPC;
+  const long long rows = (long long)ny * nz, total = rows * nx * bpp;
+  const int segs = (nx + SEG - 1) / SEG;
+  for (long long item = blockIdx.x; item < rows * segs; item += gridDim.x) {
+    const long long r = item / segs;
+    const int x0 = (int)(item % segs) * SEG, x1 = min(nx, x0 + SEG);
     const int y = (int)(r % ny), z = (int)(r / ny);
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-      const long long v = r * nx + x;
+    __syncthreads();
+    const Stages S = stage_rows<CONN>((const unsigned char *)img, total, nx, ny, nz, bpp, r, x0, x1,
+                                      w_own, w_next, w_up);
+    __syncthreads();
+    for (int x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
 #pragma unroll
       for (int j = 0; j < Fwd<CONN>::N; ++j) {
         if (!Fwd<CONN>::inside(j, x, y, z, nx, ny, nz)) continue;
-        const unsigned long long D = sqdiff<BPC>(img, C, v, v + Fwd<CONN>::off(j, nx, plane));
+        const unsigned long long D = edge_d<CONN, BPC>(S, w_own, w_next, w_up, x, j, bpp, C);
         if (CONN == 6 && j == 2 && zsep) { s1 += D; ++p1; }
         else { s0 += D; ++p0; }
       }
@@ -440,20 +544,28 @@ __global__ __launch_bounds__(256) void k_pairwise(const void *img, int C, int nx
                                                   int nz, int px, const unsigned *T,
                                                   PairParams pp, int *c0, long long stride) {
   __shared__ unsigned s_T[3 * TMAX];
+  __shared__ unsigned w_own[ROWW], w_next[ROWW], w_up[CONN == 6 ? ROWW : 1];
   for (int i = threadIdx.x; i < 3 * TMAX; i += blockDim.x) s_T[i] = T[i];
-  __syncthreads();
-  const long long rows = (long long)ny * nz, plane = (long long)nx * ny,
-                  splane = (long long)px * ny;
-  for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+  const int bpp = C * BPC;
+  const long long rows = (long long)ny * nz, total = rows * nx * bpp;
+  const long long splane = (long long)px * ny;
+  const int segs = (nx + SEG - 1) / SEG;
+  for (long long item = blockIdx.x; item < rows * segs; item += gridDim.x) {
+    const long long r = item / segs;
+    const int x0 = (int)(item % segs) * SEG, x1 = min(nx, x0 + SEG);
     const int y = (int)(r % ny), z = (int)(r / ny);
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-      const long long v = r * nx + x, sv = r * px + x;
+    __syncthreads();
+    const Stages S = stage_rows<CONN>((const unsigned char *)img, total, nx, ny, nz, bpp, r, x0, x1,
+                                      w_own, w_next, w_up);
+    __syncthreads();
+    for (int x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
+      const long long sv = r * px + x;
 #pragma unroll
       for (int j = 0; j < Fwd<CONN>::N; ++j) {
         const int d = Fwd<CONN>::dir(j);
         int w = 0;
         if (Fwd<CONN>::inside(j, x, y, z, nx, ny, nz)) {
-          const unsigned long long D = sqdiff<BPC>(img, C, v, v + Fwd<CONN>::off(j, nx, plane));
+          const unsigned long long D = edge_d<CONN, BPC>(S, w_own, w_next, w_up, x, j, bpp, C);
           const int tb = Fwd<CONN>::table(j);
           const int n = pp.n[tb];
           const unsigned *t = s_T + tb * TMAX;
@@ -565,7 +677,8 @@ int run_model(const InstDesc &I, const void *img, int C, const unsigned char *se
   }
   // unary terms from the final centres, straight into the stored rows
   const long long den = prm->range * prm->range * (1LL << (2 * KM_FRAC));
-  const int rgrid = (int)std::min<long long>(rows, 8192);
+  // one workgroup per row segment of SEG pixels, up to 8192 in flight
+  const int rgrid = (int)std::min<long long>(rows * ((I.nx + SEG - 1) / SEG), 8192);
   k_unary<BPC><<<rgrid, 256, 0, st>>>(img, C, seeds, I.nx, rows, I.px, s, prm->k_fg,
                                       prm->k_bg, den, prm->unary_scale, cs + I.node_off,
                                       ct + I.node_off);

@@ -250,12 +250,13 @@ __device__ __forceinline__ TileNode tile_node(const Tiles &T, int t, int t_end,
 // grid rows, not tiles: row r of the graph (the rows of all instances
 // concatenated) is nx consecutive nodes, so a workgroup streams it with
 // 16-byte loads and stores, several in flight per lane.  A lane handles a
-// group of up to 4 consecutive nodes: an unaligned head group, aligned
-// groups of 4, a tail group.  Per-tile flags are derived from the column and
-// the row's tile row; per-instance sums are flushed when a workgroup's row
-// changes instance and at its end.
+// group of 4 consecutive nodes (rows start at multiples of 32 nodes).  A
+// workgroup step takes one row, or several short rows of a one-instance
+// graph (RowStep), so that every lane has groups to stream.  Per-tile flags
+// are derived from the column and the row's tile row; per-instance sums are
+// flushed when a workgroup's step changes instance and at its end.
 
-// Rows per workgroup step and groups in flight per lane.
+// Groups in flight per lane.
 constexpr int RS_U = 4;
 
 template <class S>
@@ -284,27 +285,50 @@ __device__ __forceinline__ RowRef<S> row_ref(const Tiles &T, int r) {
   return R;
 }
 
+// Row r of a one-instance graph (no search over the instances).
+template <class S>
+__device__ __forceinline__ RowRef<S> row_ref_one(const Tiles &T, int r) {
+  const InstDesc &I = T.inst[0];
+  const int y = r % I.ny, z = r / I.ny;
+  RowRef<S> R;
+  R.inst = 0;
+  R.nx = I.nx;
+  R.v0 = I.node_off + (long long)r * I.px;
+  R.tile0 = I.tile_off + I.tx * (y / S::TH + I.ty * (z / S::TD));
+  return R;
+}
+
 // Group k of a row: first column x and node count n (0 past the end).
 struct Group {
   int x, n;
 };
 
-__device__ __forceinline__ Group row_group(long long v0, int nx, int k) {
-  const int head = min((int)((4 - (v0 & 3)) & 3), nx);
-  Group g;
-  if (head > 0) {
-    if (k == 0) { g.x = 0; g.n = head; return g; }
-    g.x = head + 4 * (k - 1);
-  } else {
-    g.x = 4 * k;
+// A workgroup step of a whole-graph pass: T.row_step rows from row r0 (one
+// instance), seen as slots of 4-node groups, ng per row.  Slot k is group
+// k % ng of row r0 + k / ng.  Rows start at multiples of 32 nodes, so a row's
+// groups are 4-aligned.
+template <class S>
+struct RowStep {
+  RowRef<S> R0;  // first row of the step
+  int r0, ng, slots;
+  __device__ __forceinline__ RowStep(const Tiles &T, int step) {
+    r0 = step * T.row_step;
+    R0 = row_ref<S>(T, r0);
+    ng = (R0.nx + 3) / 4;
+    slots = min(T.row_step, T.nrows - r0) * ng;
   }
-  g.n = max(0, min(4, nx - g.x));
-  return g;
-}
+  // The row and group of slot k (k < slots).
+  __device__ __forceinline__ void at(const Tiles &T, int k, RowRef<S> &R, Group &g) const {
+    const int dr = T.row_step > 1 ? k / ng : 0;
+    const int gk = T.row_step > 1 ? k - dr * ng : k;
+    R = dr ? row_ref_one<S>(T, r0 + dr) : R0;
+    g.x = 4 * gk;
+    g.n = min(4, R.nx - g.x);
+  }
+};
 
-__device__ __forceinline__ int row_groups(long long v0, int nx) {
-  const int head = min((int)((4 - (v0 & 3)) & 3), nx);
-  return (head > 0 ? 1 : 0) + (nx - head + 3) / 4;
+__device__ __forceinline__ int row_steps(const Tiles &T) {
+  return (T.nrows + T.row_step - 1) / T.row_step;
 }
 
 // Load / store up to 4 consecutive ints (16-byte access when the group is
@@ -413,26 +437,28 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
   const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
   InstSums<3> sums;
   sums.reset();
-  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
-    const RowRef<S> R = row_ref<S>(T, r);
-    sums.enter(R.inst, dst, stride, buf);
-    const int K = MASKS == 2 ? T.inst[R.inst].K : 0;
-    const int ng = row_groups(R.v0, R.nx);
-    for (int k0 = threadIdx.x; k0 < ng; k0 += U * 256) {
+  const int nsteps = row_steps(T);
+  for (int step = blockIdx.x; step < nsteps; step += gridDim.x) {
+    const RowStep<S> RS(T, step);
+    sums.enter(RS.R0.inst, dst, stride, buf);
+    const int K = MASKS == 2 ? T.inst[RS.R0.inst].K : 0;
+    for (int k0 = threadIdx.x; k0 < RS.slots; k0 += U * 256) {
       Group g[U];
+      RowRef<S> R[U];
       int4 a[U], b[U], m[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = k0 + u * 256;
-        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
-        a[u] = load4(P.cs, R.v0 + g[u].x, g[u].n);
-        b[u] = load4(P.ct, R.v0 + g[u].x, g[u].n);
-        if (MASKS == 1) m[u] = load4((const int *)mask0, R.v0 + g[u].x, g[u].n);
+        if (k < RS.slots) RS.at(T, k, R[u], g[u]);
+        else { R[u] = RS.R0; g[u] = Group{0, 0}; }
+        a[u] = load4(P.cs, R[u].v0 + g[u].x, g[u].n);
+        b[u] = load4(P.ct, R[u].v0 + g[u].x, g[u].n);
+        if (MASKS == 1) m[u] = load4((const int *)mask0, R[u].v0 + g[u].x, g[u].n);
         if (MASKS == 2) {
           int4 c[S::KMAX];
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d)
-            c[d] = d < K ? load4(P.c0 + d * P.stride, R.v0 + g[u].x, g[u].n) : make_int4(0, 0, 0, 0);
+            c[d] = d < K ? load4(P.c0 + d * P.stride, R[u].v0 + g[u].x, g[u].n) : make_int4(0, 0, 0, 0);
           m[u] = make_int4(0, 0, 0, 0);
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d) {
@@ -446,7 +472,7 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (g[u].n == 0) continue;
-        const long long v = R.v0 + g[u].x;
+        const long long v = R[u].v0 + g[u].x;
         int4 e;
         e.x = a[u].x - b[u].x; e.y = a[u].y - b[u].y;
         e.z = a[u].z - b[u].z; e.w = a[u].w - b[u].w;
@@ -463,8 +489,8 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
           bits |= (unsigned)(s > t) << i;
           dbits |= (unsigned)(s < t) << i;
         }
-        if (active) flag_group<S>(active, R.tile0, g[u], bits);
-        flag_group<S>(defi, R.tile0, g[u], dbits);
+        if (active) flag_group<S>(active, R[u].tile0, g[u], bits);
+        flag_group<S>(defi, R[u].tile0, g[u], dbits);
       }
     }
   }
@@ -477,7 +503,7 @@ void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
   if (active) launch_fill_u8(active, T.ntiles, 0, st);
   launch_fill_u8(defi, T.ntiles, 0, st);
   const int z = zero_h ? 1 : 0;
-  const int g = row_grid(T.nrows);
+  const int g = row_grid((T.nrows + T.row_step - 1) / T.row_step);
   if (masks == 2) {
     JFC_DISPATCH(shape, (k_fold<S, 2><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
   } else if (masks == 1) {
@@ -501,18 +527,20 @@ __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned ch
   const int stride[1] = {1};
   InstSums<1> sums;
   sums.reset();
-  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
-    const RowRef<S> R = row_ref<S>(T, r);
-    sums.enter(R.inst, dst, stride, buf);
-    const int ng = row_groups(R.v0, R.nx);
-    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
+  const int nsteps = row_steps(T);
+  for (int step = blockIdx.x; step < nsteps; step += gridDim.x) {
+    const RowStep<S> RS(T, step);
+    sums.enter(RS.R0.inst, dst, stride, buf);
+    for (int k0 = threadIdx.x; k0 < RS.slots; k0 += RS_U * 256) {
       Group g[RS_U];
+      RowRef<S> R[RS_U];
       int4 e[RS_U];
 #pragma unroll
       for (int u = 0; u < RS_U; ++u) {
         const int k = k0 + u * 256;
-        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
-        e[u] = load4(P.e, R.v0 + g[u].x, g[u].n);
+        if (k < RS.slots) RS.at(T, k, R[u], g[u]);
+        else { R[u] = RS.R0; g[u] = Group{0, 0}; }
+        e[u] = load4(P.e, R[u].v0 + g[u].x, g[u].n);
       }
 #pragma unroll
       for (int u = 0; u < RS_U; ++u) {
@@ -523,8 +551,8 @@ __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned ch
           sums.a[0] += max(-x, 0);
           bits |= (unsigned)(x > 0) << i;
         }
-        store4_u8(P.lab, R.v0 + g[u].x, g[u].n, bits);
-        flag_group<S>(seed, R.tile0, g[u], bits);
+        store4_u8(P.lab, R[u].v0 + g[u].x, g[u].n, bits);
+        flag_group<S>(seed, R[u].tile0, g[u], bits);
       }
     }
   }
@@ -536,7 +564,8 @@ void launch_cut_init(int shape, const Tiles &T, const Planes &P,
   launch_fill_u8(seed, T.ntiles, 0, st);
   launch_fill_i32((int *)def, 2LL * T.ninst, 0, st);
   JFC_DISPATCH(shape,
-               (k_cut_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, seed, def)));
+               (k_cut_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0, st>>>(
+                   T, P, seed, def)));
 }
 
 // graphcut_init_global_relabel (graphcut_global_relabel.cl:19-48) as a pass
@@ -548,17 +577,19 @@ template <class S>
 __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
                                                  unsigned char *defi) {
   const int zero = tag_word(0, tag), inf = tag_word(HINF, tag);
-  for (int r = blockIdx.x; r < T.nrows; r += gridDim.x) {
-    const RowRef<S> R = row_ref<S>(T, r);
-    const int ng = row_groups(R.v0, R.nx);
-    for (int k0 = threadIdx.x; k0 < ng; k0 += RS_U * 256) {
+  const int nsteps = row_steps(T);
+  for (int step = blockIdx.x; step < nsteps; step += gridDim.x) {
+    const RowStep<S> RS(T, step);
+    for (int k0 = threadIdx.x; k0 < RS.slots; k0 += RS_U * 256) {
       Group g[RS_U];
+      RowRef<S> R[RS_U];
       int4 e[RS_U];
 #pragma unroll
       for (int u = 0; u < RS_U; ++u) {
         const int k = k0 + u * 256;
-        g[u] = k < ng ? row_group(R.v0, R.nx, k) : Group{0, 0};
-        e[u] = load4(P.e, R.v0 + g[u].x, g[u].n);
+        if (k < RS.slots) RS.at(T, k, R[u], g[u]);
+        else { R[u] = RS.R0; g[u] = Group{0, 0}; }
+        e[u] = load4(P.e, R[u].v0 + g[u].x, g[u].n);
       }
 #pragma unroll
       for (int u = 0; u < RS_U; ++u) {
@@ -566,10 +597,10 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
         int4 h;
         h.x = e[u].x < 0 ? zero : inf; h.y = e[u].y < 0 ? zero : inf;
         h.z = e[u].z < 0 ? zero : inf; h.w = e[u].w < 0 ? zero : inf;
-        store4(P.h, R.v0 + g[u].x, g[u].n, h);
+        store4(P.h, R[u].v0 + g[u].x, g[u].n, h);
         unsigned bits = 0;
         for (int i = 0; i < g[u].n; ++i) bits |= (unsigned)(comp(e[u], i) < 0) << i;
-        flag_group<S>(defi, R.tile0, g[u], bits);
+        flag_group<S>(defi, R[u].tile0, g[u], bits);
       }
     }
   }
@@ -578,7 +609,8 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
 void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
                     hipStream_t st) {
   launch_fill_u8(defi, T.ntiles, 0, st);
-  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid(T.nrows), 256, 0, st>>>(T, P, tag, defi)));
+  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0,
+                                       st>>>(T, P, tag, defi)));
 }
 
 // Cut energy on the input capacities (QGCWidget.cpp:2452-2520), summed per
@@ -1051,11 +1083,13 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
 // sweep, and decide on the device what the host would decide after reading
 // the count, so that the host can queue sweeps ahead of its checks:
 //   stop = 1: no tile was active (converged, or the certificate decides);
-//   stop = 2: a global relabel is due -- the active set neither shrank nor
-//             grew by more than (1 - stall_ratio) over the last sweep since
-//             the last relabel, or the tile visits since it exceed the
-//             budget (the work of relabel_work visits per tile);
-//   stop = 3: the sweep budget is spent.
+//   stop = 2: the active set neither shrank nor grew by more than
+//             (1 - stall_ratio) over the last sweep since the last relabel
+//             (a global relabel is due, or the remaining excess is cut off
+//             from every deficit);
+//   stop = 3: the sweep budget is spent;
+//   stop = 4: the tile visits since the last relabel exceed the budget (the
+//             work of relabel_work visits per tile): a relabel is due.
 // A sweep that starts with stop set compacts nothing and files nothing.
 __global__ void k_sweep_end(SweepCtl c) {
   int *st = c.state;
@@ -1077,7 +1111,8 @@ __global__ void k_sweep_end(SweepCtl c) {
                          (double)cnt >= c.stall_ratio * prev && cnt * c.stall_ratio <= (double)prev;
     const long long done = c.total[0] - *(const long long *)(st + 4);
     const bool budget = c.budget > 0 && (double)done > (double)c.budget;
-    if (stalled || budget) stop = 2;
+    if (stalled) stop = 2;
+    else if (budget) stop = 4;
   }
   st[2] = stop;
 }

@@ -39,7 +39,7 @@ class Options(ctypes.Structure):
                 ("stream", ctypes.c_void_p), ("initial_relabel", ctypes.c_int),
                 ("stall_ratio", ctypes.c_double), ("graph_sweeps", ctypes.c_int),
                 ("tile_order", ctypes.c_int), ("method", ctypes.c_int),
-                ("sparse_local_iters", ctypes.c_int)]
+                ("sparse_local_iters", ctypes.c_int), ("stall_certify", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -101,7 +101,8 @@ EXPORTS = ["jfc_default_options", "jfc_create", "jfc_create_batch", "jfc_destroy
            "jfc_graph_size", "jfc_debug_tile_flags", "jfc_mfi_info", "jfc_mfi_read",
            "jfc_mfi_write", "jfc_raw_read", "jfc_raw_write", "jfc_cut_read", "jfc_cut_write",
            "jfc_node_read", "jfc_node_write", "jfc_mfi_export", "jfc_get_tags",
-           "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_get_initial_relabel", "jfc_default_widget_model_params",
+           "jfc_autotune", "jfc_get_tile_shape", "jfc_get_local_iters", "jfc_get_initial_relabel",
+           "jfc_get_tuning", "jfc_default_widget_model_params",
            "jfc_model_widget", "jfc_set_stream", "jfc_input_planes", "jfc_inputs_written",
            "jfc_model_image_static"]
 
@@ -167,6 +168,7 @@ def lib():
         "jfc_get_tile_shape": (I, [P, ctypes.POINTER(I)]),
         "jfc_get_local_iters": (I, [P, ctypes.POINTER(I)]),
         "jfc_get_initial_relabel": (I, [P, ctypes.POINTER(I)]),
+        "jfc_get_tuning": (I, [P, ctypes.POINTER(Options)]),
         "jfc_default_widget_model_params": (None, [ctypes.POINTER(WidgetModelParams)]),
         "jfc_model_widget": (I, [P, I, P, P, P, I, ctypes.POINTER(WidgetModelParams),
                                  ctypes.POINTER(WidgetModelStats)]),
@@ -467,6 +469,16 @@ class GridGraph:
         ch = ctypes.c_int()
         _check(lib().jfc_get_initial_relabel(self._h, ctypes.byref(ch)))
         return ch.value
+
+    def tuning(self):
+        """The choices default solves use (jfc_get_tuning): dict of
+        tile_shape (name), max_local_iters, initial_relabel, stall_certify,
+        sparse_local_iters."""
+        o = default_options()
+        _check(lib().jfc_get_tuning(self._h, ctypes.byref(o)))
+        return {"tile_shape": TILE_NAME[o.tile_shape], "max_local_iters": o.max_local_iters,
+                "initial_relabel": o.initial_relabel, "stall_certify": o.stall_certify,
+                "sparse_local_iters": o.sparse_local_iters}
 
     def size(self):
         """(total nodes over all instances, largest connectivity)."""
