@@ -107,10 +107,11 @@ typedef struct {
   int sparse_local_iters; /* cap on LDS discharge iterations per visit in a
                              push launch whose active-tile list is no longer
                              than its grid (each workgroup visits at most one
-                             tile): 0 (default) = the cap jfc_autotune chose
-                             for the graph's size class, else 32; negative =
-                             max_local_iters (the reference loops each visit
-                             until no push, graphcut_push_relabel.cl:77-182) */
+                             tile), from the solve's 8th sweep on: 0 (default)
+                             = the cap jfc_autotune chose for the graph's size
+                             class, else 32; negative = max_local_iters (the
+                             reference loops each visit until no push,
+                             graphcut_push_relabel.cl:77-182) */
   int stall_certify;      /* when the active set stalls (stall_ratio): 1 = run
                              the canonical-cut search first and finish if it
                              finds no augmenting path, else relabel; 0 =
@@ -364,7 +365,7 @@ int jfc_model_image(jfc_graph *g, int inst, const void *image, int channels,
  * must be serialised by the caller.  Results are identical to
  * jfc_model_image's. */
 #define JFC_MODEL_STATIC_MAX_SEEDS (1 << 22)
-#define JFC_MODEL_STATIC_BYTES (136448 + 4 * JFC_MODEL_STATIC_MAX_SEEDS)
+#define JFC_MODEL_STATIC_BYTES (666112 + 4 * JFC_MODEL_STATIC_MAX_SEEDS)
 int jfc_model_image_static(jfc_graph *g, int inst, const void *image, int channels,
                            int bytes_per_channel, const uint8_t *seeds,
                            const jfc_model_params *params, jfc_model_stats *stats);

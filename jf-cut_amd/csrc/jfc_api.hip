@@ -96,10 +96,11 @@ struct Tuned {
   int sp = 0;   // cap of sparse launches (0: not tuned)
 };
 
-// Default cap on the local iterations of a launch whose list fits its grid
-// (the reference fixtures: 4.6 -> 2.2-3.0 ms against a cap of 2; larger caps
-// slow graphs whose few active tiles iterate without effect, config 1 0.48
-// -> 0.71 ms at 64).
+// Default cap on the local iterations of a launch whose list fits its grid,
+// from the solve's kSparseAfter-th sweep on (the reference fixtures: 4.6 ->
+// 2.2-3.0 ms against a cap of 2; from the first sweep on, larger caps slowed
+// graphs whose few active tiles iterate without effect, config 1 0.48 ->
+// 0.71 ms at 64).
 constexpr int kSparseIters = 32;
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tune;
@@ -184,6 +185,10 @@ struct jfc_graph {
   int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
   hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
   hipEvent_t priv_event = nullptr;
+  // Second stream of the data term (its pair sums and pairwise weights run
+  // beside the k-means and the unary terms), with the fork and join events.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t aux_fork = nullptr, aux_join = nullptr;
   // The caller's stream for every call of the graph (jfc_set_stream): the
   // setters, the data term, the outputs and the backward issue their work
   // on it and wait for that stream alone.  NULL: the legacy default stream.
@@ -272,6 +277,11 @@ struct jfc_graph {
     if (priv_stream) (void)hipStreamDestroy(priv_stream);
     priv_event = nullptr;
     priv_stream = nullptr;
+    if (aux_fork) (void)hipEventDestroy(aux_fork);
+    if (aux_join) (void)hipEventDestroy(aux_join);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
+    aux_fork = aux_join = nullptr;
+    aux_stream = nullptr;
     for (hipEvent_t &ev : cnt_ev) {
       if (ev) (void)hipEventDestroy(ev);
       ev = nullptr;
@@ -454,6 +464,28 @@ int graph_scratch(jfc_graph *g, size_t bytes, void **p) {
     g->model_scratch_bytes = bytes;
   }
   *p = g->model_scratch;
+  return JFC_OK;
+}
+
+// The data term's second stream and its fork / join events: created on first
+// use when `create` is set; otherwise null until then (the caller then runs
+// on one stream).
+int graph_aux(jfc_graph *g, bool create, hipStream_t *st, hipEvent_t *fork, hipEvent_t *join) {
+  if (!g->aux_stream && create) {
+    if (hipStreamCreateWithFlags(&g->aux_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g->aux_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->aux_join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      if (g->aux_fork) (void)hipEventDestroy(g->aux_fork);
+      if (g->aux_stream) (void)hipStreamDestroy(g->aux_stream);
+      g->aux_fork = g->aux_join = nullptr;
+      g->aux_stream = nullptr;
+      return fail(JFC_ERR_DEVICE, "cannot create the data term's second stream");
+    }
+  }
+  *st = g->aux_stream;
+  *fork = g->aux_fork;
+  *join = g->aux_join;
   return JFC_OK;
 }
 
@@ -913,8 +945,10 @@ struct Solver {
     c.hist = g->d_sweep;
     c.state = g->d_sweep_state;
     c.total = g->d_total;
+    c.iters = g->d_total + kIterSlots;
     c.cap = g->sweep_cap;
     c.max_sweeps = o.max_sweeps;
+    c.iter_unit = o.max_local_iters;
     c.stall_ratio = o.stall_ratio;
     c.budget = o.relabel_work > 0 ? (long long)std::floor(o.relabel_work * g->ntiles) : 0;
     return c;
@@ -1064,7 +1098,7 @@ struct Solver {
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
                   g->d_touched, g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,
-                  g->d_total + kIterSlots, s_);
+                  g->d_sweep_state, g->d_total + kIterSlots, s_);
       if (timed && o.profile) push_ev.push_back({a, tm.mark()});
       advance();
     }

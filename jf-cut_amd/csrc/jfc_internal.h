@@ -166,8 +166,10 @@ struct SweepCtl {
   int *hist;             // active tiles per sweep
   int *state;
   const long long *total;  // tile visits of the push kernel so far
+  const long long *iters;  // its local iterations so far, spread over 64 words
   int cap;               // entries of hist
   int max_sweeps;
+  int iter_unit;         // local iterations that count as one visit (the dense cap)
   double stall_ratio;
   long long budget;      // tile visits between relabels (0 = no budget)
 };
@@ -192,14 +194,17 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // this solve (by a push visit, which then writes all of them, or by a global
 // relabel); heights of a tile with hmat 0 read as 0, the labelling a solve
 // starts from.  max_iters caps the local iterations of a visit, sparse_iters
-// those of a launch whose list fits its grid.  iter_total: 64 int64 counters
+// those of a launch whose list fits its grid once kSparseAfter sweeps ran.  iter_total: 64 int64 counters
 // of local iterations (or null).
 // touched: per tile, 1 once a flush wrote into the tile's planes or mask
 // words (cleared when a solve starts).
+// sweep_state: the solve's SweepCtl state ([1] = sweep index).
+constexpr int kSparseAfter = 8;  // sweeps before the sparse cap applies
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
                  unsigned char *touched, unsigned char *defi, unsigned char *hmat,
-                 int max_iters, int sparse_iters, long long *iter_total, hipStream_t st);
+                 int max_iters, int sparse_iters, const int *sweep_state,
+                 long long *iter_total, hipStream_t st);
 // Fast-Cut comparator (jfc_fastcut.hip)
 void launch_fc_materialize(const Planes &P, const unsigned *mask0, int K, hipStream_t st);
 void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,

@@ -1109,7 +1109,14 @@ __global__ void k_sweep_end(SweepCtl c) {
     const int prev = since >= 1 && i >= 1 ? c.hist[i - 1] : 0;
     const bool stalled = c.stall_ratio > 0.0 && since >= 1 &&
                          (double)cnt >= c.stall_ratio * prev && cnt * c.stall_ratio <= (double)prev;
-    const long long done = c.total[0] - *(const long long *)(st + 4);
+    // work since the relabel: tile visits, or local iterations over the
+    // dense cap where sparse launches iterate their visits longer
+    long long done = c.total[0] - *(const long long *)(st + 4);
+    if (c.budget > 0) {
+      long long it = 0;
+      for (int k = 0; k < 64; ++k) it += c.iters[k];
+      done = max(done, (it - *(const long long *)(st + 6)) / max(1, c.iter_unit));
+    }
     const bool budget = c.budget > 0 && (double)done > (double)c.budget;
     if (stalled) stop = 2;
     else if (budget) stop = 4;
@@ -1119,13 +1126,17 @@ __global__ void k_sweep_end(SweepCtl c) {
 
 void launch_sweep_end(const SweepCtl &c, hipStream_t st) { k_sweep_end<<<1, 1, 0, st>>>(c); }
 
-// After a global relabel: sweeping resumes, and the relabel's sweep index
-// and visit total are the base of the next stall and budget tests.
+// After a global relabel: sweeping resumes, and the relabel's sweep index,
+// visit total and iteration total are the base of the next stall and budget
+// tests.
 __global__ void k_sweep_resume(SweepCtl c) {
   int *st = c.state;
   st[2] = 0;
   st[3] = st[1];
   *(long long *)(st + 4) = c.total[0];
+  long long it = 0;
+  for (int k = 0; k < 64; ++k) it += c.iters[k];
+  *(long long *)(st + 6) = it;
 }
 
 void launch_sweep_resume(const SweepCtl &c, hipStream_t st) {
@@ -1326,9 +1337,10 @@ constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 // their LDS slots and is flushed to HBM once per visit.  The loop ends when a
 // whole iteration pushes nothing (graphcut_push_relabel.cl:77-182) or after
 // max_iters iterations -- sparse_iters when the launch's list is no longer
-// than its grid (every workgroup holds at most one tile, so a visit that
-// keeps iterating delays no other tile, and the flow crosses the tile in one
-// visit instead of one sweep per step).
+// than its grid and the solve has run kSparseAfter sweeps (every workgroup
+// holds at most one tile, so a visit that keeps iterating delays no other
+// tile, and the flow crosses the tile in one visit instead of one sweep per
+// step).
 template <class S>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
 // without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
@@ -1337,7 +1349,8 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
                                               unsigned char *flag, unsigned char *mat,
                                               unsigned char *touched, unsigned char *defi,
                                               unsigned char *hmat, int max_iters,
-                                              int sparse_iters, long long *iter_total) {
+                                              int sparse_iters, const int *sweep_state,
+                                              long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
@@ -1347,7 +1360,10 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
   __shared__ int s_w[8];
   int par = 0;
   QueueCursor Qc(q, &s_run);
-  const int cap_iters = Qc.count <= (int)gridDim.x ? sparse_iters : max_iters;
+  // the sparse cap from the solve's kSparseAfter-th sweep on: a solve that
+  // ends within a few sweeps has nothing to gain from long visits
+  const int cap_iters =
+      Qc.count <= (int)gridDim.x && sweep_state[1] >= kSparseAfter ? sparse_iters : max_iters;
   const long long st = P.stride;
   long long my_iters = 0;
   int idx = -1, end = 0;
@@ -1517,6 +1533,38 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     } while (any && ++it < cap_iters);
     my_iters += it + (any ? 0 : 1);
 
+    // ---- the flush's reads first: the flow that crossed into each ring cell
+    // and the reverse-residual increments (LDS), and for increments into a
+    // plane that is still fresh (its value lives in c0) the c0 value they add
+    // to, loaded before the interior write-back so that its stores cover the
+    // load latency.
+    int fl[kRingPerLane<S>];
+    int dc[kRingPerLane<S>][S::KMAX], cv[kRingPerLane<S>][S::KMAX];
+    unsigned fr[kRingPerLane<S>], fp[kRingPerLane<S>];
+#pragma unroll
+    for (int c = 0; c < kRingPerLane<S>; ++c) {
+      fl[c] = 0;
+      fr[c] = fp[c] = 0;
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) dc[c][d] = cv[c][d] = 0;
+      if (!hc[c].live) continue;
+      const int p = hp[c];
+      fl[c] = s_e[p];
+#pragma unroll
+      for (int d = 0; d < S::KMAX; ++d) {
+        dc[c][d] = d < K ? s_c[d][p] : 0;
+        if (dc[c][d] != 0) {
+          if ((hm[c] >> (8 + d)) & 1u) fr[c] |= 1u << d;
+          if (!((hm[c] >> d) & 1u)) fp[c] |= 1u << d;
+        }
+      }
+      if (fr[c]) {
+        const long long g = hc[c].g;
+#pragma unroll
+        for (int d = 0; d < S::KMAX; ++d) cv[c][d] = P.c0[((fr[c] >> d) & 1u) ? d * st + g : 0];
+      }
+    }
+
     // ---- write back what changed (stores only)
     int act = 0, dfc = 0;
     if (valid) {
@@ -1543,45 +1591,28 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     // Plane d of halo node g is the arc from g into this tile, so no other
     // tile of this colour class touches plane d of g or its two mask bits in
     // this launch; other tiles may update other bits of the same mask word,
-    // hence atomics.  A plane that is still fresh (its value lives in c0) is
-    // written as c0 + dc; the others take a fire-and-forget atomic add.
+    // hence atomics.  A plane that is still fresh is written as c0 + dc; the
+    // others take a fire-and-forget atomic add.
 #pragma unroll
     for (int c = 0; c < kRingPerLane<S>; ++c) {
       if (!hc[c].live) continue;
       const int p = hp[c];
       const long long g = hc[c].g;
-      const int f = s_e[p];
-      if (f != 0) {
+      if (fl[c] != 0) {
         // flow entered g, so some plane of g changes below: g's tile is no
         // longer clean
-        atomicAdd(&P.e[g], f);
+        atomicAdd(&P.e[g], fl[c]);
         const int ht = tile_of<S>(I, r.ox + (p % S::PW) - 1, r.oy + (p / S::PW) % S::PH - 1,
                                   S::D3 ? r.oz + p / (S::PW * S::PH) - 1 : 0);
         flag[ht] = 1;
         touched[ht] = 1;
       }
-      int dc[S::KMAX];
-      unsigned fresh = 0, flip = 0;
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) {
-        dc[d] = d < K ? s_c[d][p] : 0;
-        if (dc[d] != 0) {
-          if ((hm[c] >> (8 + d)) & 1u) fresh |= 1u << d;
-          if (!((hm[c] >> d) & 1u)) flip |= 1u << d;
-        }
+        if ((fr[c] >> d) & 1u) P.cap[d * st + g] = cv[c][d] + dc[c][d];
+        else if (dc[c][d] != 0) atomicAdd(&P.cap[d * st + g], dc[c][d]);
       }
-      flip |= fresh << 8;
-      if (fresh) {
-        int cv[S::KMAX];
-#pragma unroll
-        for (int d = 0; d < S::KMAX; ++d) cv[d] = P.c0[((fresh >> d) & 1u) ? d * st + g : 0];
-#pragma unroll
-        for (int d = 0; d < S::KMAX; ++d)
-          if ((fresh >> d) & 1u) P.cap[d * st + g] = cv[d] + dc[d];
-      }
-#pragma unroll
-      for (int d = 0; d < S::KMAX; ++d)
-        if (dc[d] != 0 && !((fresh >> d) & 1u)) atomicAdd(&P.cap[d * st + g], dc[d]);
+      const unsigned flip = fp[c] | (fr[c] << 8);
       if (flip) atomicXor(&P.rmask[g], flip);
     }
     // the tile's active flag and deficit flag, one barrier for both
@@ -1601,10 +1632,11 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
 void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  const Queue &q, unsigned char *flag, unsigned char *mat,
                  unsigned char *touched, unsigned char *defi, unsigned char *hmat,
-                 int max_iters, int sparse_iters, long long *iter_total, hipStream_t st) {
+                 int max_iters, int sparse_iters, const int *sweep_state,
+                 long long *iter_total, hipStream_t st) {
   JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
                           T, P, q, flag, mat, touched, defi, hmat, max_iters, sparse_iters,
-                          iter_total)));
+                          sweep_state, iter_total)));
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
