@@ -188,7 +188,8 @@ struct jfc_graph {
   // Second stream of the data term (its pair sums and pairwise weights run
   // beside the k-means and the unary terms), with the fork and join events.
   hipStream_t aux_stream = nullptr;
-  hipEvent_t aux_fork = nullptr, aux_join = nullptr;
+  hipEvent_t aux_fork = nullptr, aux_join = nullptr, aux_ready = nullptr;
+  void *aux_pin = nullptr;  // pinned host words the second stream's copies use
   // The caller's stream for every call of the graph (jfc_set_stream): the
   // setters, the data term, the outputs and the backward issue their work
   // on it and wait for that stream alone.  NULL: the legacy default stream.
@@ -262,6 +263,16 @@ struct jfc_graph {
     dfree(d_touched);
     dfree(d_lines);
   }
+  void release_aux() {
+    if (aux_fork) (void)hipEventDestroy(aux_fork);
+    if (aux_join) (void)hipEventDestroy(aux_join);
+    if (aux_ready) (void)hipEventDestroy(aux_ready);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
+    if (aux_pin) (void)hipHostFree(aux_pin);
+    aux_fork = aux_join = aux_ready = nullptr;
+    aux_stream = nullptr;
+    aux_pin = nullptr;
+  }
   void release() {
     release_tiles();
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
@@ -277,11 +288,7 @@ struct jfc_graph {
     if (priv_stream) (void)hipStreamDestroy(priv_stream);
     priv_event = nullptr;
     priv_stream = nullptr;
-    if (aux_fork) (void)hipEventDestroy(aux_fork);
-    if (aux_join) (void)hipEventDestroy(aux_join);
-    if (aux_stream) (void)hipStreamDestroy(aux_stream);
-    aux_fork = aux_join = nullptr;
-    aux_stream = nullptr;
+    release_aux();
     for (hipEvent_t &ev : cnt_ev) {
       if (ev) (void)hipEventDestroy(ev);
       ev = nullptr;
@@ -467,25 +474,26 @@ int graph_scratch(jfc_graph *g, size_t bytes, void **p) {
   return JFC_OK;
 }
 
-// The data term's second stream and its fork / join events: created on first
-// use when `create` is set; otherwise null until then (the caller then runs
-// on one stream).
-int graph_aux(jfc_graph *g, bool create, hipStream_t *st, hipEvent_t *fork, hipEvent_t *join) {
+// The data term's second stream, its fork / join / ready events and its
+// pinned host words (`pin_bytes`): created on first use when `create` is
+// set; otherwise null until then (the caller then runs on one stream).
+int graph_aux(jfc_graph *g, bool create, size_t pin_bytes, GraphAux *a) {
   if (!g->aux_stream && create) {
     if (hipStreamCreateWithFlags(&g->aux_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&g->aux_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&g->aux_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&g->aux_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->aux_ready, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(&g->aux_pin, pin_bytes, hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
-      if (g->aux_fork) (void)hipEventDestroy(g->aux_fork);
-      if (g->aux_stream) (void)hipStreamDestroy(g->aux_stream);
-      g->aux_fork = g->aux_join = nullptr;
-      g->aux_stream = nullptr;
+      g->release_aux();
       return fail(JFC_ERR_DEVICE, "cannot create the data term's second stream");
     }
   }
-  *st = g->aux_stream;
-  *fork = g->aux_fork;
-  *join = g->aux_join;
+  a->st = g->aux_stream;
+  a->fork = g->aux_fork;
+  a->join = g->aux_join;
+  a->ready = g->aux_ready;
+  a->pin = g->aux_pin;
   return JFC_OK;
 }
 

@@ -47,6 +47,13 @@ struct InstDesc {
   long long user_off;
 };
 
+// The data term's second stream of a graph (null members: not created).
+struct GraphAux {
+  hipStream_t st;
+  hipEvent_t fork, join, ready;
+  void *pin;  // pinned host words
+};
+
 struct Planes {
   int *e;            // excess (> 0) / deficit (< 0), folded terminal arcs
   int *h;            // height word (HMASK bits); HINF = cannot reach a deficit

@@ -47,7 +47,7 @@ hipStream_t graph_stream(jfc_graph *g);
 int graph_stage(jfc_graph *g, size_t bytes, void **p);
 int graph_scratch(jfc_graph *g, size_t bytes, void **p);
 int graph_work_plane(jfc_graph *g, int inst, int **plane, long long *n);
-int graph_aux(jfc_graph *g, bool create, hipStream_t *st, hipEvent_t *fork, hipEvent_t *join);
+int graph_aux(jfc_graph *g, bool create, size_t pin_bytes, GraphAux *a);
 }  // namespace jfc
 
 using namespace jfc;
@@ -89,6 +89,8 @@ constexpr int KM_GRID = 256;  // assigning workgroups per side, at most
 constexpr size_t kPartBytes = align256(4 * KM_GRID * KMAX_CLUSTERS * 4 * 8);
 constexpr int NSUM = 256;
 constexpr size_t kSumsBytes = align256(NSUM * 2 * 8);
+// pinned words of the second stream: the sum slots, then the thresholds
+constexpr size_t kAuxPinBytes = NSUM * 2 * 8 + 3 * TMAX * 4;
 constexpr size_t kTBytes = align256(3 * TMAX * 4);
 constexpr size_t kCtlBytes = align256(8 * 4);
 constexpr size_t kSmallBytes =
@@ -140,7 +142,7 @@ __global__ void k_model_clear(Scratch s, int iters) {
 // pixels) and adds the non-zero bins to the global histograms; 16-bit images
 // keep per key the sum of the 4 bits below the key (value = 16 key + low
 // bits).
-constexpr int kScanGrid = 1024;
+constexpr int kScanGrid = 1536;  // 6 workgroups per CU (LDS-bound)
 
 template <int BPC, int C>
 __global__ __launch_bounds__(256) void k_seed_scan(const void *img, const unsigned char *seeds,
@@ -1117,7 +1119,8 @@ int check_model_params(int channels, int bpc, const jfc_model_params *prm) {
 // stream's work; every return after the fork goes through it.
 struct Streams {
   hipStream_t st, aux;
-  hipEvent_t fork, join_ev;
+  hipEvent_t fork, join_ev, ready;
+  void *pin;  // pinned host words of the second stream's copies
   bool forked = false;
   hipStream_t side() const { return aux ? aux : st; }
   hipError_t fork_now() {
@@ -1168,13 +1171,7 @@ int run_model(const InstDesc &I, const void *img, const unsigned char *seeds,
   // kItemsPerBlock row segments of SEG pixels per workgroup
   const int rgrid = (int)((rows * ((I.nx + SEG - 1) / SEG) + kItemsPerBlock - 1) / kItemsPerBlock);
   k_model_clear<<<64, 256, 0, st>>>(s, iters);
-  // the seed scan first: its workgroups are resident before the pair sums'
-  MODEL_HIP(S.fork_now());
   k_seed_scan<BPC, C><<<grid1((n + 15) / 16, kScanGrid), 256, 0, st>>>(img, seeds, n, s);
-  const hipStream_t sb = S.side();
-  if (I.conn == 6) launch_pair_sums<6, BPC, C>(rgrid, I, img, zsep, s, sb);
-  else if (I.conn == 8) launch_pair_sums<8, BPC, C>(rgrid, I, img, 0, s, sb);
-  else launch_pair_sums<4, BPC, C>(rgrid, I, img, 0, s, sb);
   int ctl[6];
   MODEL_HIP(hipMemcpyAsync(ctl, s.ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   MODEL_HIP(hipStreamSynchronize(st));
@@ -1187,6 +1184,21 @@ int run_model(const InstDesc &I, const void *img, const unsigned char *seeds,
                                            std::to_string((long long)ctl[0] + ctl[1]) +
                                            ") than the seed list holds (" +
                                            std::to_string(s.cap) + ")");
+  // the pair sums on the second stream from here on: launched earlier, they
+  // would hold the copy of the seed counts back behind their workgroups
+  MODEL_HIP(S.fork_now());
+  const hipStream_t sb = S.side();
+  if (I.conn == 6) launch_pair_sums<6, BPC, C>(rgrid, I, img, zsep, s, sb);
+  else if (I.conn == 8) launch_pair_sums<8, BPC, C>(rgrid, I, img, 0, s, sb);
+  else launch_pair_sums<4, BPC, C>(rgrid, I, img, 0, s, sb);
+  // with a second stream, its sums come back to pinned words while the
+  // host issues the k-means and the unary terms
+  long long host_slots[NSUM * 2];
+  long long *slots = S.aux ? (long long *)S.pin : host_slots;
+  if (S.aux) {
+    MODEL_HIP(hipMemcpyAsync(slots, s.sums, NSUM * 2 * sizeof(long long), hipMemcpyDeviceToHost, sb));
+    MODEL_HIP(hipEventRecord(S.ready, sb));
+  }
   // k-means on the device: iteration `it` assigns and updates; a side whose
   // centres stop moving is done and its later launches return at once
   k_km_init<<<2, 256, 0, st>>>(s, C, nkeys, prm->k_fg, prm->k_bg);
@@ -1212,10 +1224,13 @@ int run_model(const InstDesc &I, const void *img, const unsigned char *seeds,
                                           cs + I.node_off, ct + I.node_off);
   *planes_written = true;
   // the thresholds on the host, from the pair sums
-  static_assert(NSUM * 2 * 8 <= 8192, "slots fit the host buffer");
-  long long slots[NSUM * 2], sums[4] = {0, 0, 0, 0};
-  MODEL_HIP(hipMemcpyAsync(slots, s.sums, sizeof(slots), hipMemcpyDeviceToHost, sb));
-  MODEL_HIP(hipStreamSynchronize(sb));
+  long long sums[4] = {0, 0, 0, 0};
+  if (S.aux) {
+    MODEL_HIP(hipEventSynchronize(S.ready));
+  } else {
+    MODEL_HIP(hipMemcpyAsync(slots, s.sums, NSUM * 2 * sizeof(long long), hipMemcpyDeviceToHost, sb));
+    MODEL_HIP(hipStreamSynchronize(sb));
+  }
   for (int i = 0; i < NSUM; ++i) {
     sums[0] += slots[2 * i];
     sums[2] += slots[2 * i + 1];
@@ -1227,8 +1242,10 @@ int run_model(const InstDesc &I, const void *img, const unsigned char *seeds,
   ST.pair_sum[0] = sums[0]; ST.pair_count[0] = sums[1];
   ST.pair_sum[1] = sums[2]; ST.pair_count[1] = sums[3];
   static_assert(sizeof(unsigned) == 4, "thresholds are 32-bit");
-  unsigned Tab[3 * TMAX];
-  std::memset(Tab, 0, sizeof(Tab));
+  unsigned host_tab[3 * TMAX];
+  // the thresholds cross from pinned words after the slots (read by now)
+  unsigned *Tab = S.aux ? (unsigned *)((char *)S.pin + NSUM * 2 * sizeof(long long)) : host_tab;
+  std::memset(Tab, 0, 3 * TMAX * sizeof(unsigned));
   PairParams pp;
   const double lam[3] = {prm->lambda_xy, prm->lambda_xy / std::sqrt(2.0), prm->lambda_z};
   const int zg = zsep ? 2 : 0;  // sums of the z table: own group or the x/y one
@@ -1238,7 +1255,7 @@ int run_model(const InstDesc &I, const void *img, const unsigned char *seeds,
     pp.lam[t] = (float)lam[t];
     pp.inv2s2[t] = (Sg[t] == 0 || Pg[t] == 0) ? 0.0f : (float)((double)Pg[t] / (2.0 * (double)Sg[t]));
   }
-  MODEL_HIP(hipMemcpyAsync(s.T, Tab, sizeof(Tab), hipMemcpyHostToDevice, sb));
+  MODEL_HIP(hipMemcpyAsync(s.T, Tab, 3 * TMAX * sizeof(unsigned), hipMemcpyHostToDevice, sb));
   int *pc0 = c0 + I.node_off;
   if (I.conn == 6) launch_pairwise<6, BPC, C>(rgrid, I, img, s, pp, pc0, stride, sb);
   else if (I.conn == 8) launch_pairwise<8, BPC, C>(rgrid, I, img, s, pp, pc0, stride, sb);
@@ -1272,7 +1289,13 @@ int model_common(jfc_graph *g, int inst, const void *img, int C, int bpc,
   bool written = false;
   Streams S;
   S.st = graph_stream(g);
-  if ((rc = graph_aux(g, create_aux, &S.aux, &S.fork, &S.join_ev))) return rc;
+  GraphAux A;
+  if ((rc = graph_aux(g, create_aux, kAuxPinBytes, &A))) return rc;
+  S.aux = A.st;
+  S.fork = A.fork;
+  S.join_ev = A.join;
+  S.ready = A.ready;
+  S.pin = A.pin;
   if (bpc == 2) rc = run_model<2, 1>(I, img, seeds, prm, ST, s, cs, ct, c0, stride, S, &written);
   else if (C == 3) rc = run_model<1, 3>(I, img, seeds, prm, ST, s, cs, ct, c0, stride, S, &written);
   else rc = run_model<1, 1>(I, img, seeds, prm, ST, s, cs, ct, c0, stride, S, &written);
