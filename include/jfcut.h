@@ -87,8 +87,10 @@ typedef struct {
                              = the choice jfc_autotune made for the graph's
                              size class, else 0 */
   double stall_ratio;     /* run a global relabel when the active-tile count of a
-                             sweep stays within [r, 1/r] of the previous sweep's
-                             (the active set has stopped changing; 0 = off) */
+                             sweep stays within [r, 1/r] of the previous sweep's,
+                             or two sweeps running repeat the two before them
+                             (the active set has stopped changing, or trades
+                             places between two sets of tiles; 0 = off) */
   int graph_sweeps;       /* 1 = replay each sweep's launches (compaction and
                              push per colour class) as one captured hipGraph
                              (default); 0 = launch them one by one.  profile = 1

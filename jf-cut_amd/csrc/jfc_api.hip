@@ -953,10 +953,8 @@ struct Solver {
     c.hist = g->d_sweep;
     c.state = g->d_sweep_state;
     c.total = g->d_total;
-    c.iters = g->d_total + kIterSlots;
     c.cap = g->sweep_cap;
     c.max_sweeps = o.max_sweeps;
-    c.iter_unit = o.max_local_iters;
     c.stall_ratio = o.stall_ratio;
     c.budget = o.relabel_work > 0 ? (long long)std::floor(o.relabel_work * g->ntiles) : 0;
     return c;

@@ -173,10 +173,8 @@ struct SweepCtl {
   int *hist;             // active tiles per sweep
   int *state;
   const long long *total;  // tile visits of the push kernel so far
-  const long long *iters;  // its local iterations so far, spread over 64 words
   int cap;               // entries of hist
   int max_sweeps;
-  int iter_unit;         // local iterations that count as one visit (the dense cap)
   double stall_ratio;
   long long budget;      // tile visits between relabels (0 = no budget)
 };

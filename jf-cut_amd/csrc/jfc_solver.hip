@@ -1106,17 +1106,19 @@ __global__ void k_sweep_end(SweepCtl c) {
     stop = 3;
   } else {
     const int since = i - st[3];  // sweeps since the last relabel, before this one
-    const int prev = since >= 1 && i >= 1 ? c.hist[i - 1] : 0;
-    const bool stalled = c.stall_ratio > 0.0 && since >= 1 &&
-                         (double)cnt >= c.stall_ratio * prev && cnt * c.stall_ratio <= (double)prev;
-    // work since the relabel: tile visits, or local iterations over the
-    // dense cap where sparse launches iterate their visits longer
-    long long done = c.total[0] - *(const long long *)(st + 4);
-    if (c.budget > 0) {
-      long long it = 0;
-      for (int k = 0; k < 64; ++k) it += c.iters[k];
-      done = max(done, (it - *(const long long *)(st + 6)) / max(1, c.iter_unit));
+    // stalled: the active count repeats that of the sweep before, or two
+    // sweeps running repeat the two before them (excess trading places
+    // between two sets of tiles, sweep after sweep)
+    auto near = [&](int a, int b) {
+      return (double)a >= c.stall_ratio * b && a * c.stall_ratio <= (double)b;
+    };
+    bool stalled = false;
+    if (c.stall_ratio > 0.0 && since >= 1 && i < c.cap) {
+      stalled = near(cnt, c.hist[i - 1]);
+      if (!stalled && since >= 3)
+        stalled = near(cnt, c.hist[i - 2]) && near(c.hist[i - 1], c.hist[i - 3]);
     }
+    const long long done = c.total[0] - *(const long long *)(st + 4);
     const bool budget = c.budget > 0 && (double)done > (double)c.budget;
     if (stalled) stop = 2;
     else if (budget) stop = 4;
@@ -1126,17 +1128,13 @@ __global__ void k_sweep_end(SweepCtl c) {
 
 void launch_sweep_end(const SweepCtl &c, hipStream_t st) { k_sweep_end<<<1, 1, 0, st>>>(c); }
 
-// After a global relabel: sweeping resumes, and the relabel's sweep index,
-// visit total and iteration total are the base of the next stall and budget
-// tests.
+// After a global relabel: sweeping resumes, and the relabel's sweep index
+// and visit total are the base of the next stall and budget tests.
 __global__ void k_sweep_resume(SweepCtl c) {
   int *st = c.state;
   st[2] = 0;
   st[3] = st[1];
   *(long long *)(st + 4) = c.total[0];
-  long long it = 0;
-  for (int k = 0; k < 64; ++k) it += c.iters[k];
-  *(long long *)(st + 6) = it;
 }
 
 void launch_sweep_resume(const SweepCtl &c, hipStream_t st) {
