@@ -416,6 +416,10 @@ def main():
     ap.add_argument("--no-fixtures", action="store_true")
     ap.add_argument("--no-seeds-only", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="tile_shape,max_local_iters,initial_relabel,stall_certify,"
+                         "sparse_local_iters: solve the headline workload with these options "
+                         "instead of autotuning (profiling runs repeat a bench run's choice)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -442,18 +446,25 @@ def main():
     t0 = time.time()
     g = build_graph(wl, seed=0)
     log(f"workload built in {time.time() - t0:.1f}s")
-    if not args.no_autotune:
-        g.autotune(reps=2)
-    tuning = g.tuning()
+    topt = {}
+    if args.tuning:
+        sh, mli, ir, sc, sp = args.tuning.split(",")
+        topt = dict(tile_shape=sh, max_local_iters=int(mli), initial_relabel=int(ir),
+                    stall_certify=int(sc), sparse_local_iters=int(sp))
+        tuning = dict(topt)
+    else:
+        if not args.no_autotune:
+            g.autotune(reps=2)
+        tuning = g.tuning()
     log(f"tuning: {tuning}")
     for i in range(args.warmup):
         g.inputs_written()
-        st = g.solve()
+        st = g.solve(**topt)
         log(f"warmup {i}: {st.ms_total:.2f} ms, sweeps {st.sweeps}, flow {st.flow}")
 
     # headline: fresh inputs, push launches timed with HIP events on the solve
     # stream (profile = 1 launches each sweep eagerly)
-    ms_per_step, stats = timed_steps(g, args.steps, fresh=True, profile=1)
+    ms_per_step, stats = timed_steps(g, args.steps, fresh=True, profile=1, **topt)
     for i, st in enumerate(stats):
         log(f"step {i}: {st.ms_total:.2f} ms device, push {st.ms_push:.2f} ms, "
             f"sweeps {st.sweeps}, relabels {st.global_relabels}, fresh {st.inputs_fresh}")
@@ -470,13 +481,13 @@ def main():
     dev_ms = sum(s.ms_total for s in stats) / len(stats)
 
     # the same solve without events (captured sweep graphs), fresh inputs
-    plain_ms, _ = timed_steps(g, args.steps, fresh=True)
+    plain_ms, _ = timed_steps(g, args.steps, fresh=True, **topt)
     # warm re-solve: unchanged inputs, masks kept from the previous solve
-    warm_ms, wst = timed_steps(g, args.steps, fresh=False)
+    warm_ms, wst = timed_steps(g, args.steps, fresh=False, **topt)
     assert all(s.inputs_fresh == 0 for s in wst)
     # inputs copied in from caller device buffers (validation included)
     setter, bufs = device_setter(g, N, K)
-    set_ms, _ = timed_steps(g, max(2, args.steps // 2), fresh=False, setter=setter)
+    set_ms, _ = timed_steps(g, max(2, args.steps // 2), fresh=False, setter=setter, **topt)
     for b in bufs:
         b.free()
     log(f"fresh {ms_per_step:.3f} ms (no events {plain_ms:.3f}), warm {warm_ms:.3f} ms, "
