@@ -112,6 +112,9 @@ int log2_round(double v) { return v <= 1.0 ? 0 : (int)std::lround(std::log2(v));
 // iterations of the push kernel (workgroup b adds to slot b % 64, so no
 // single address takes one atomic per workgroup).
 constexpr int kIterSlots = 5, kTotals = kIterSlots + 64;
+// words of the results block besides the per-instance ones: the counters,
+// the sweep state (kSweepState ints) and 4 flag words
+constexpr size_t kResFixed = kTotals + (kSweepState + 4) / 2;
 
 // Sweeps whose stop words can be in flight at once (pinned words and events).
 constexpr int kStopRing = 8;
@@ -163,8 +166,13 @@ struct jfc_graph {
   unsigned char *d_touched = nullptr;  // per tile: planes or masks written by a flush
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
-  long long *d_red = nullptr, *d_total = nullptr;
-  long long *d_def = nullptr;  // per instance: remaining deficit after the cut
+  // One device block holds every word a solve reads back at its end (one
+  // copy): the per-instance sums, the remaining deficits, the counters, the
+  // sweep state and the flag words, in this order (res_words long longs).
+  long long *d_res = nullptr;
+  size_t res_words = 0;
+  long long *d_red = nullptr, *d_total = nullptr;  // in d_res
+  long long *d_def = nullptr;  // in d_res; per instance: remaining deficit after the cut
   long long *d_energy = nullptr;  // per instance: cut energy (jfc_get_energy)
   int *d_row_off = nullptr;    // per instance: first row; [count] = rows
   int *d_lines = nullptr;      // per axis: prefix count of tile lines per instance
@@ -173,7 +181,7 @@ struct jfc_graph {
   int *d_sweep = nullptr;
   int sweep_cap = 0;
   int last_sweeps = 0;
-  int *d_flagword = nullptr;  // [0] certificate, [1] validation
+  int *d_flagword = nullptr;  // in d_res; [0] certificate, [1] validation
   unsigned *mask0 = nullptr;  // residual-arc bits of the input planes
   unsigned *rmask = nullptr;  // residual-arc bits of the current residual
   bool mask_dirty = true;     // inputs changed since mask0 was built
@@ -182,7 +190,7 @@ struct jfc_graph {
   std::vector<long long> flows;
   std::vector<hipEvent_t> ev_pool;
   int nwg = 2048;
-  int *d_sweep_state = nullptr;     // {active tiles of the running sweep, sweep index}
+  int *d_sweep_state = nullptr;     // in d_res; {active tiles of the running sweep, sweep index}
   hipStream_t priv_stream = nullptr;  // solve stream when the caller passes NULL
   hipEvent_t priv_event = nullptr;
   // Second stream of the data term (its pair sums and pairwise weights run
@@ -202,13 +210,13 @@ struct jfc_graph {
     int recent[3];    // active tiles of the latest sweeps
     int flag;         // certificate word of a cut search
     int bad;          // validation word of a setter
-    int sweeps;       // sweeps run (the device's count)
-    int stopw[kStopRing];  // stop words of the sweeps in flight
     long long red[RED_SLOTS];  // one instance's sums (energy)
   } *pin = nullptr;
-  // Pinned copies of a solve's results: per-instance sums (RED_SLOTS each),
-  // remaining deficits, then the kTotals counters.
+  // Pinned copy of d_res (the solve's results).
   long long *pin_res = nullptr;
+  // Stop words the sweeps post (k_sweep_end), in fine-grained pinned memory
+  // the device writes through to; d_ring is its device address.
+  int *ring = nullptr, *d_ring = nullptr;
   hipEvent_t cnt_ev[2] = {nullptr, nullptr};
   hipEvent_t stop_ev[kStopRing] = {};
   int ensure_pin() {
@@ -223,13 +231,19 @@ struct jfc_graph {
         ev = nullptr;
         return fail(JFC_ERR_DEVICE, "event creation failed");
       }
-    const size_t nres = inst.size() * (RED_SLOTS + 1) + kTotals;
     if (hipHostMalloc((void **)&pin, sizeof(HostPin), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&pin_res, nres * sizeof(long long), hipHostMallocDefault) !=
-            hipSuccess) {
+        hipHostMalloc((void **)&pin_res, res_words * sizeof(long long), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&ring, kStopRing * sizeof(int), hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void **)&d_ring, ring, 0) != hipSuccess) {
+      (void)hipGetLastError();
       if (pin) (void)hipHostFree(pin);
+      if (pin_res) (void)hipHostFree(pin_res);
+      if (ring) (void)hipHostFree(ring);
       pin = nullptr;
       pin_res = nullptr;
+      ring = d_ring = nullptr;
       return fail(JFC_ERR_OOM, "pinned host words");
     }
     return JFC_OK;
@@ -276,8 +290,9 @@ struct jfc_graph {
   void release() {
     release_tiles();
     dfree(cs); dfree(ct); dfree(c0); dfree(cap); dfree(e); dfree(h); dfree(lab); dfree(tag);
-    dfree(d_ctr); dfree(d_red); dfree(d_total); dfree(d_sweep); dfree(d_flagword);
-    dfree(d_def); dfree(d_energy); dfree(d_row_off); dfree(d_sweep_state);
+    dfree(d_ctr); dfree(d_res); dfree(d_sweep); dfree(d_energy); dfree(d_row_off);
+    d_red = d_total = d_def = nullptr;
+    d_sweep_state = d_flagword = nullptr;
     if (stage) (void)hipFree(stage);
     stage = nullptr;
     stage_bytes = 0;
@@ -299,8 +314,10 @@ struct jfc_graph {
     }
     if (pin) (void)hipHostFree(pin);
     if (pin_res) (void)hipHostFree(pin_res);
+    if (ring) (void)hipHostFree(ring);
     pin = nullptr;
     pin_res = nullptr;
+    ring = d_ring = nullptr;
     dfree(mask0); dfree(rmask);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     ev_pool.clear();
@@ -311,7 +328,8 @@ struct jfc_graph {
     // a one-instance graph with short rows takes several rows per step
     int step = 1;
     if (inst.size() == 1) step = std::max(1, 1024 / ((inst[0].nx + 3) / 4));
-    return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows, step};
+    return Tiles{d_inst, d_tile_inst, ntiles, d_row_off, (int)inst.size(), nrows, step,
+                 inst.empty() ? InstDesc{} : inst[0]};
   }
   Planes planes() const { return Planes{e, h, cap, c0, cs, ct, lab, Ntot, rmask}; }
   TileLines lines() const {
@@ -581,16 +599,18 @@ static int create_impl(jfc_graph **out, int count, const int32_t *shapes) {
       (rc = dalloc(&g->lab, N, "labels")) || (rc = dalloc(&g->d_ctr, 2 * QSLOT, "queues")) ||
       (rc = dalloc(&g->mask0, N, "residual masks")) ||
       (rc = dalloc(&g->rmask, N, "residual masks")) ||
-      (rc = dalloc(&g->d_red, (size_t)count * RED_SLOTS, "reductions")) ||
-      (rc = dalloc(&g->d_total, kTotals, "counters")) ||
-      (rc = dalloc(&g->d_def, (size_t)count, "reductions")) ||
+      (rc = dalloc(&g->d_res, kResFixed + (size_t)count * (RED_SLOTS + 1), "results")) ||
       (rc = dalloc(&g->d_energy, (size_t)count, "reductions")) ||
-      (rc = dalloc(&g->d_sweep_state, kSweepState, "sweep counters")) ||
-      (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table")) ||
-      (rc = dalloc(&g->d_flagword, 4, "flags"))) {
+      (rc = dalloc(&g->d_row_off, (size_t)count + 1, "row table"))) {
     delete g;
     return rc;
   }
+  g->res_words = kResFixed + (size_t)count * (RED_SLOTS + 1);
+  g->d_red = g->d_res;
+  g->d_def = g->d_red + (size_t)count * RED_SLOTS;
+  g->d_total = g->d_def + count;
+  g->d_sweep_state = (int *)(g->d_total + kTotals);
+  g->d_flagword = g->d_sweep_state + kSweepState;
   {
     std::vector<int> row_off(count + 1, 0);
     long long rows = 0;
@@ -957,6 +977,8 @@ struct Solver {
     c.max_sweeps = o.max_sweeps;
     c.stall_ratio = o.stall_ratio;
     c.budget = o.relabel_work > 0 ? (long long)std::floor(o.relabel_work * g->ntiles) : 0;
+    c.ring = g->d_ring;
+    c.ring_n = kStopRing;
     return c;
   }
 
@@ -976,7 +998,9 @@ struct Solver {
   // tag) in jump-flooding distance order; later passes visit the dirty tiles,
   // alternating direction (back-and-forth, QGCWidget.cpp:2221-2240).  Tile
   // visits are counted on the device (d_total) and read once per solve.
-  int search(int mode, int *passes) {
+  // cleared: the caller has zeroed the dirty flags and the distance
+  // histogram already (search_fills).
+  int search(int mode, int *passes, bool cleared = false) {
     const Tiles T = g->tiles();
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
@@ -984,8 +1008,8 @@ struct Solver {
     // fold and the push visits); cut and tag seeds: d_seed from their init
     launch_tile_order(g->shape, T, mode == kRelabel ? g->d_defi : g->d_seed, g->d_jf0, g->d_jf1,
                       g->d_tdist, g->d_hist, g->nbins, g->d_order, g->max_tile_dim, nullptr,
-                      g->lines(), o.tile_order, st);
-    launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
+                      g->lines(), o.tile_order, st, cleared);
+    if (!cleared) launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
@@ -1058,12 +1082,25 @@ struct Solver {
   // exact: 3D tiles have a halo as large as the tile, and without the pass
   // (halo excess read in the first tile pass, seeds from the push visits'
   // deficit flags) config 3 took 14.7 ms instead of 13.4.
+  // The zeroing a search needs (dirty flags, distance histogram), to go into
+  // its caller's fill launch.
+  void search_fills(FillSet &f) const {
+    f.add(g->d_hist, g->nbins, 0, 4);
+    f.add(g->d_dirty, g->ntiles, 0, 1);
+  }
+
   int global_relabel() {
     hipEvent_t a = tm.mark();
     g->gr_tag = 3 - g->gr_tag;
+    {
+      FillSet f;
+      search_fills(f);
+      if (g->dim == 3) f.add(g->d_defi, g->ntiles, 0, 1);
+      launch_fill_multi(f, st);
+    }
     if (g->dim == 3)
-      launch_gr_init(g->shape, g->tiles(), g->planes(), g->gr_tag, g->d_defi, st);
-    int rc = search(kRelabel, &s.relabel_passes);
+      launch_gr_init(g->shape, g->tiles(), g->planes(), g->gr_tag, g->d_defi, st, true);
+    int rc = search(kRelabel, &s.relabel_passes, true);
     if (rc) return rc;
     launch_fill_u8(g->d_hmat, g->ntiles, 1, st);  // every height is stored now
     gr_ev.push_back({a, tm.mark()});
@@ -1074,18 +1111,32 @@ struct Solver {
   int cuts = 0;  // canonical-cut searches run
 
   // Returns 1 in *found when an augmenting path still exists.
+  hipEvent_t last_cut_end = nullptr;
+
+  // Returns 1 in *found when an augmenting path still exists.  The search is
+  // the solve's last step when it finds none, so the certificate word comes
+  // back with every result of the solve, in one copy of the results block.
   int canonical_cut(int *found) {
     hipEvent_t a = tm.mark();
     ++cuts;
-    launch_fill_i32(g->d_flagword, 1, 0, st);
-    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, st);
-    int rc = search(kCut, &s.cut_passes);
+    {
+      FillSet f;
+      f.add(g->d_flagword, 1, 0, 4);
+      f.add(g->d_seed, g->ntiles, 0, 1);
+      f.add(g->d_def, 2LL * (long long)g->inst.size(), 0, 4);
+      search_fills(f);
+      launch_fill_multi(f, st);
+    }
+    launch_cut_init(g->shape, g->tiles(), g->planes(), g->d_seed, g->d_def, st, true);
+    int rc = search(kCut, &s.cut_passes, true);
     if (rc) return rc;
-    cut_ev.push_back({a, tm.mark()});
-    JFC_HIP(hipMemcpyAsync(&g->pin->flag, g->d_flagword, sizeof(int), hipMemcpyDeviceToHost, st));
+    last_cut_end = tm.mark();
+    cut_ev.push_back({a, last_cut_end});
+    JFC_HIP(hipMemcpyAsync(g->pin_res, g->d_res, g->res_words * sizeof(long long),
+                           hipMemcpyDeviceToHost, st));
     JFC_HIP(hipEventRecord(g->cnt_ev[0], st));
     JFC_HIP(hipEventSynchronize(g->cnt_ev[0]));
-    *found = g->pin->flag;
+    *found = *((const int *)g->pin_res + ((const int *)g->d_flagword - (const int *)g->d_res));
     return JFC_OK;
   }
 
@@ -1188,6 +1239,9 @@ struct Solver {
       f.add(g->d_mat, g->ntiles, 0, 1);
       f.add(g->d_hmat, g->ntiles, 0, 1);
       f.add(g->d_touched, g->ntiles, 0, 1);
+      // the fold's tile flags
+      if (!o.initial_relabel) f.add(g->d_flag, g->ntiles, 0, 1);
+      f.add(g->d_defi, g->ntiles, 0, 1);
       launch_fill_multi(f, st);
     }
     g->slot = 0;
@@ -1206,7 +1260,7 @@ struct Solver {
     s.solve_bytes += nodes * (12 + (masks == 2 ? 4 * K + 8 : 8));
     if (int rc0 = g->ensure_pin()) return rc0;
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi,
-                o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, g->mask0, masks, st);
+                o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, g->mask0, masks, st, true);
     // Fast-Cut reads and writes the residual planes directly
     if (o.method == JFC_METHOD_FASTCUT) launch_fc_materialize(P, g->mask0, g->Kg, st);
     // the fold's sums are read with the results (no host wait here): an
@@ -1225,10 +1279,13 @@ struct Solver {
     const int ahead = std::min(std::max(1, o.check_every), kStopRing - 1);
     int cert_rounds = 0;
     bool done = false;
-    int issued = 0;  // sweeps issued in this solve, no-ops included
+    // sweeps issued in this solve (no-ops included; the device numbers them
+    // the same way), and the stop words the host has read or skipped
+    int sent = 0, known = 0;
+    // no word of an earlier solve matches a sequence number of this one
+    for (int k = 0; k < kStopRing; ++k) g->ring[k] = -1;
     for (;;) {
       int stopw = 0;
-      int sent = 0, known = 0;
       for (;;) {
         while (sent < known + ahead) {
           // the first sweep of a solve runs eagerly (it also primes the
@@ -1237,26 +1294,33 @@ struct Solver {
             issue_fast_sweep();
             s.push_launches += 2 + 2 * g->dim;
           } else {
-            if (o.graph_sweeps && !o.profile && issued > 0) {
+            if (o.graph_sweeps && !o.profile && sent > 0) {
               if ((rc = replay_sweep())) return rc;
             } else {
               issue_sweep(st, true);
             }
             s.push_launches += g->ncolors;
           }
-          const int k = sent % kStopRing;
-          JFC_HIP(hipMemcpyAsync(&g->pin->stopw[k], g->d_sweep_state + 2, sizeof(int),
-                                 hipMemcpyDeviceToHost, st));
-          JFC_HIP(hipEventRecord(g->stop_ev[k], st));
+          JFC_HIP(hipEventRecord(g->stop_ev[sent % kStopRing], st));
           ++sent;
-          ++issued;
         }
         const int k = known % kStopRing;
         JFC_HIP(hipEventSynchronize(g->stop_ev[k]));
-        stopw = g->pin->stopw[k];
+        // the sweep's end kernel has completed, so its word is in the ring
+        // (written through to host memory); re-read a few times before
+        // giving up on a word that does not carry the expected number
+        const volatile int *ring = g->ring;
+        int w = ring[k];
+        for (int spin = 0; (w >> 3) != known && spin < (1 << 20); ++spin) w = ring[k];
+        if ((w >> 3) != known)
+          return fail(JFC_ERR_DEVICE, "sweep " + std::to_string(known) +
+                                          ": its stop word did not reach the host");
+        stopw = w & 7;
         ++known;
         if (stopw) break;
       }
+      // the sweeps queued behind the stop are no-ops: skip their words
+      known = sent;
       // the sweeps queued behind the stop were no-ops
       if (stopw == 3) break;  // sweep budget spent
       if (stopw == 2 && o.stall_certify > 0) {
@@ -1293,19 +1357,12 @@ struct Solver {
       int found = 0;
       if ((rc = canonical_cut(&found))) return rc;
     }
-    // the last cut search ran on the final excess and left the remaining
-    // deficit of every instance in d_def
-    hipEvent_t t1 = tm.mark();
-    long long *pred = g->pin_res, *pdef = pred + red.size(), *totals = pdef + nI;
-    JFC_HIP(hipMemcpyAsync(pred, g->d_red, red.size() * sizeof(long long),
-                           hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipMemcpyAsync(pdef, g->d_def, nI * sizeof(long long), hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipMemcpyAsync(totals, g->d_total, kTotals * sizeof(long long),
-                           hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipMemcpyAsync(&g->pin->sweeps, g->d_sweep_state + 1, sizeof(int),
-                           hipMemcpyDeviceToHost, st));
-    JFC_HIP(hipStreamSynchronize(st));
-    s.sweeps = g->pin->sweeps;
+    // the last cut search ran on the final excess, left the remaining deficit
+    // of every instance in d_def and copied the results block back (the
+    // solve's device work ends there)
+    hipEvent_t t1 = last_cut_end;
+    const long long *pred = g->pin_res, *pdef = pred + red.size(), *totals = pdef + nI;
+    s.sweeps = ((const int *)(totals + kTotals))[1];
     std::copy(pred, pred + red.size(), red.begin());
     const std::vector<long long> def(pdef, pdef + nI);
     for (int i = 0; i < nI; ++i)

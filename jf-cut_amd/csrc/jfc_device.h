@@ -65,10 +65,9 @@ struct TileRef {
 };
 
 template <class S>
-__device__ __forceinline__ TileRef tile_ref(const Tiles &T, int t) {
+__device__ __forceinline__ TileRef tile_ref_of(const InstDesc &I, int t) {
   TileRef r;
-  // one instance: the descriptor does not depend on t (no dependent load)
-  r.I = T.ninst == 1 ? T.inst[0] : T.inst[T.tile_inst[t]];
+  r.I = I;
   const int lt = t - r.I.tile_off;
   r.ltx = lt % r.I.tx;
   const int q = lt / r.I.tx;
@@ -79,6 +78,13 @@ __device__ __forceinline__ TileRef tile_ref(const Tiles &T, int t) {
   r.oz = r.ltz * S::TD;
   return r;
 }
+
+template <class S>
+__device__ __forceinline__ TileRef tile_ref(const Tiles &T, int t) {
+  // one instance: the descriptor does not depend on t (no dependent load)
+  return tile_ref_of<S>(T.ninst == 1 ? T.inst[0] : T.inst[T.tile_inst[t]], t);
+}
+
 
 __device__ __forceinline__ long long node_index(const InstDesc &I, int x, int y,
                                                 int z) {

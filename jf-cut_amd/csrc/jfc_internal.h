@@ -76,6 +76,8 @@ struct Tiles {
   int nrows;             // rows of nx nodes over all instances
   int row_step;          // rows a workgroup streams at once in the whole-graph
                          // passes (several short rows of a one-instance graph)
+  InstDesc one;          // the descriptor of instance 0 (kernel argument: a
+                         // one-instance visit reads it with scalar loads)
 };
 
 // Tile shapes (256 nodes each, one node per lane).
@@ -119,10 +121,11 @@ struct FillSeg {
   int bytes;
 };
 struct FillSet {
-  FillSeg seg[8];
+  static constexpr int kMax = 12;
+  FillSeg seg[kMax];
   int n = 0;
   void add(void *p, long long count, int v, int bytes) {
-    if (count > 0 && n < 8) seg[n++] = FillSeg{p, count, v, bytes};
+    if (count > 0 && n < kMax) seg[n++] = FillSeg{p, count, v, bytes};
   }
 };
 void launch_fill_multi(const FillSet &f, hipStream_t st);
@@ -134,9 +137,10 @@ void launch_fill_multi(const FillSet &f, hipStream_t st);
 // visited in the solve as 0, see launch_push).
 // masks: 0 = rmask untouched; 1 = rmask = mask0; 2 = build mask0 from the
 // input planes and write it to rmask too (all in the same pass).
+// cleared: the caller has zeroed active and defi already.
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
                  unsigned char *active, unsigned char *defi, bool zero_h, unsigned *mask0,
-                 int masks, hipStream_t st);
+                 int masks, hipStream_t st, bool cleared = false);
 void launch_validate(const int *p, long long n, int lo, int hi, int *bad,
                      hipStream_t st);
 void launch_zero_border(int shape, const InstDesc &I, int *plane, int d,
@@ -147,10 +151,10 @@ void launch_seeds(int *cs, int *ct, const unsigned char *fg, const unsigned char
 void launch_excess_split(const int *ef, int *cs, int *ct, long long n,
                          hipStream_t st);
 
-// Also sums the remaining deficit max(-e, 0) per instance into def[inst]
-// (def must be zeroed by the caller).
+// Also sums the remaining deficit max(-e, 0) per instance into def[inst].
+// seed and def are zeroed here, unless the caller has (cleared).
 void launch_cut_init(int shape, const Tiles &T, const Planes &P,
-                     unsigned char *seed, long long *def, hipStream_t st);
+                     unsigned char *seed, long long *def, hipStream_t st, bool cleared = false);
 
 // Jump-flooding tile distance + counting sort of tiles by distance.
 // Lines of the tile grids along each axis: prefix[a][i] = lines along axis a
@@ -159,16 +163,19 @@ struct TileLines {
   const int *prefix[3];
   int total[3];
 };
+// hist_cleared: the caller has zeroed hist (nbins ints) already.
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *jf0, int *jf1, int *tdist, int *hist, int nbins,
                        int *order, int max_tile_dim, int *max_dist,
-                       const TileLines &lines, int jump_flooding, hipStream_t st);
+                       const TileLines &lines, int jump_flooding, hipStream_t st,
+                       bool hist_cleared = false);
 
 // Sweep bookkeeping on the device.  state (8 ints): [0] active tiles of the
 // running sweep, [1] sweep index, [2] stop word (0 sweep on, 1 no active
 // tile, 2 active set stalled, 3 sweep budget spent, 4 relabel budget
 // spent), [3] sweep index of the last
-// relabel, [4..5] tile visits (int64) at the last relabel.
+// relabel, [4..5] tile visits (int64) at the last relabel, [6] sweeps ended
+// (no-ops included) in the solve.
 struct SweepCtl {
   int *hist;             // active tiles per sweep
   int *state;
@@ -177,6 +184,10 @@ struct SweepCtl {
   int max_sweeps;
   double stall_ratio;
   long long budget;      // tile visits between relabels (0 = no budget)
+  // Pinned host ring the end of sweep s posts s * 8 + stop into, at slot
+  // s % ring_n (null: no posting).
+  int *ring;
+  int ring_n;
 };
 constexpr int kSweepState = 8;
 void launch_sweep_end(const SweepCtl &c, hipStream_t st);
@@ -217,7 +228,7 @@ void launch_fc_sweep(int shape, int nwg, const Tiles &T, const Planes &P, const 
 // Relabel start values written over all nodes with `tag`, and the exact
 // deficit flags of the tiles (3D relabels).
 void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
-                    hipStream_t st);
+                    hipStream_t st, bool cleared = false);
 // One relabel pass.  The first pass (first_pass, every tile) starts each
 // tile from e (0 at deficits, HINF elsewhere) and writes every height word
 // with `tag`; later passes, and every pass after launch_gr_init, relax the

@@ -499,9 +499,11 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
 
 void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
                  unsigned char *active, unsigned char *defi, bool zero_h, unsigned *mask0,
-                 int masks, hipStream_t st) {
-  if (active) launch_fill_u8(active, T.ntiles, 0, st);
-  launch_fill_u8(defi, T.ntiles, 0, st);
+                 int masks, hipStream_t st, bool cleared) {
+  if (!cleared) {
+    if (active) launch_fill_u8(active, T.ntiles, 0, st);
+    launch_fill_u8(defi, T.ntiles, 0, st);
+  }
   const int z = zero_h ? 1 : 0;
   const int g = row_grid((T.nrows + T.row_step - 1) / T.row_step);
   if (masks == 2) {
@@ -560,9 +562,11 @@ __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned ch
 }
 
 void launch_cut_init(int shape, const Tiles &T, const Planes &P,
-                     unsigned char *seed, long long *def, hipStream_t st) {
-  launch_fill_u8(seed, T.ntiles, 0, st);
-  launch_fill_i32((int *)def, 2LL * T.ninst, 0, st);
+                     unsigned char *seed, long long *def, hipStream_t st, bool cleared) {
+  if (!cleared) {
+    launch_fill_u8(seed, T.ntiles, 0, st);
+    launch_fill_i32((int *)def, 2LL * T.ninst, 0, st);
+  }
   JFC_DISPATCH(shape,
                (k_cut_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0, st>>>(
                    T, P, seed, def)));
@@ -607,8 +611,8 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
 }
 
 void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
-                    hipStream_t st) {
-  launch_fill_u8(defi, T.ntiles, 0, st);
+                    hipStream_t st, bool cleared) {
+  if (!cleared) launch_fill_u8(defi, T.ntiles, 0, st);
   JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0,
                                        st>>>(T, P, tag, defi)));
 }
@@ -982,7 +986,8 @@ __global__ __launch_bounds__(256) void k_scatter_order(const int *tdist, int n, 
 void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
                        int *jf0, int *jf1, int *tdist, int *hist, int nbins,
                        int *order, int max_tile_dim, int *max_dist,
-                       const TileLines &lines, int jump_flooding, hipStream_t st) {
+                       const TileLines &lines, int jump_flooding, hipStream_t st,
+                       bool hist_cleared) {
   const int n = T.ntiles;
   const int g = (n + 255) / 256;
   const int d3 = shape_is_3d(shape) ? 1 : 0;
@@ -1015,7 +1020,7 @@ void launch_tile_order(int shape, const Tiles &T, const unsigned char *seed,
     launch_fill_i32(max_dist, 1, 0, st);
     k_max_dist<<<g, 256, 0, st>>>(tdist, n, nbins - 1, max_dist);
   }
-  launch_fill_i32(hist, nbins, 0, st);
+  if (!hist_cleared) launch_fill_i32(hist, nbins, 0, st);
   const int gh = std::min(g, 512);
   k_hist<<<gh, 256, nbins * sizeof(int), st>>>(tdist, n, nbins, hist);
   k_scan_1wg<<<1, 1024, 0, st>>>(hist, nbins);
@@ -1091,9 +1096,23 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
 //   stop = 4: the tile visits since the last relabel exceed the budget (the
 //             work of relabel_work visits per tile): a relabel is due.
 // A sweep that starts with stop set compacts nothing and files nothing.
+// Every sweep, no-op or not, posts its stop word to the host ring tagged with
+// its sequence number in the solve, so that the host reads it with no copy
+// and tells it from an older word in the same slot.
+__device__ __forceinline__ void post_stop(const SweepCtl &c, int seq, int stop) {
+  if (c.ring)
+    __hip_atomic_store(c.ring + seq % c.ring_n, seq * 8 + stop, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_sweep_end(SweepCtl c) {
   int *st = c.state;
-  if (st[2]) return;
+  const int seq = st[6];
+  st[6] = seq + 1;
+  if (st[2]) {
+    post_stop(c, seq, st[2]);
+    return;
+  }
   const int i = st[1];
   const int cnt = st[0];
   if (i < c.cap) c.hist[i] = cnt;
@@ -1124,6 +1143,7 @@ __global__ void k_sweep_end(SweepCtl c) {
     else if (budget) stop = 4;
   }
   st[2] = stop;
+  post_stop(c, seq, stop);
 }
 
 void launch_sweep_end(const SweepCtl &c, hipStream_t st) { k_sweep_end<<<1, 1, 0, st>>>(c); }
@@ -1218,6 +1238,81 @@ struct QueueCursor {
     if (ok && threadIdx.x == 0)
       ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[QWORD * head_id], 1) : -1;
     return ok;
+  }
+};
+
+// The cursor of k_push: QueueCursor's runs and claims, but a workgroup learns
+// its next list entry while it discharges the current tile, so that a visit
+// starts with neither a global load nor a barrier of its own.  Thread 0 works
+// out the next position during the local iterations (the rest of the current
+// run, else the run its claim obtained, in which case it claims the run after
+// that at once) and loads its entry; it publishes position and entry in LDS
+// before the visit's last barrier, after which every thread reads them.
+struct PushCursor {
+  int count, chunk, head_id, nheads;
+  int *heads;
+  const int *list;
+  int ahead;             // (thread 0) claim for the run after the current one
+  int nidx, nend, nent;  // (thread 0) the next position, its run's end, its entry
+  int *s;                // LDS: next position, run end, entry
+  int idx, end, ent;     // current position (uniform; idx < 0: drained)
+  __device__ __forceinline__ PushCursor(const Queue &q, int *lds3) {
+    if (blockIdx.x == 0 && threadIdx.x < 9) q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
+    count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
+    chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
+    heads = q.ctr + QSLOT * q.slot + QWORD;
+    list = q.list;
+    s = lds3;
+    const int b = blockIdx.x;
+    nheads = (gridDim.x % 8 == 0) ? 8 : 1;
+    head_id = b % nheads;
+    int first = b;
+    if (gridDim.x % 64 == 0) {
+      const int w = b >> 3;
+      first = (((w >> 3) << 3) + (b & 7)) * 8 + (w & 7);
+    }
+    idx = first * chunk;
+    end = min(idx + chunk, count);
+    ahead = -1;
+    nidx = -1;
+    nend = nent = 0;
+    if (idx < count) {
+      ent = __builtin_amdgcn_readfirstlane(list[idx]);
+      if (threadIdx.x == 0 && (long long)gridDim.x * chunk < count)
+        ahead = atomicAdd(&heads[QWORD * head_id], 1);
+    } else {
+      idx = -1;
+      ent = 0;
+    }
+  }
+  // thread 0, during a visit: the position after the current one, and its entry
+  __device__ __forceinline__ void prefetch() {
+    nidx = -1;
+    if (idx + 1 < end) {
+      nidx = idx + 1;
+      nend = end;
+    } else if (ahead >= 0) {
+      const int k = ahead;  // k-th run handed out by this head
+      const int run = (int)gridDim.x + 8 * (head_id + nheads * (k >> 3)) + (k & 7);
+      if ((long long)run * chunk < count) {
+        nidx = run * chunk;
+        nend = min(nidx + chunk, count);
+        ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[QWORD * head_id], 1) : -1;
+      }
+    }
+    if (nidx >= 0) nent = list[nidx];
+  }
+  // thread 0, after prefetch() and before the visit's last barrier
+  __device__ __forceinline__ void publish() const {
+    s[0] = nidx;
+    s[1] = nend;
+    s[2] = nent;
+  }
+  // every thread, after the visit's last barrier
+  __device__ __forceinline__ void advance() {
+    idx = __builtin_amdgcn_readfirstlane(s[0]);
+    end = __builtin_amdgcn_readfirstlane(s[1]);
+    ent = __builtin_amdgcn_readfirstlane(s[2]);
   }
 };
 
@@ -1339,7 +1434,9 @@ constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 // holds at most one tile, so a visit that keeps iterating delays no other
 // tile, and the flow crosses the tile in one visit instead of one sweep per
 // step).
-template <class S>
+// ONE: a one-instance graph, whose descriptor every visit reads from the
+// kernel arguments instead of loading it behind the tile index.
+template <class S, bool ONE>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
 // without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
 // 8, where it spills).
@@ -1354,21 +1451,20 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
   __shared__ int s_c[S::KMAX][S::PADN];
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  __shared__ int s_run;
+  __shared__ int s_next[3];
   __shared__ int s_w[8];
   int par = 0;
-  QueueCursor Qc(q, &s_run);
+  PushCursor Qc(q, s_next);
   // the sparse cap from the solve's kSparseAfter-th sweep on: a solve that
   // ends within a few sweeps has nothing to gain from long visits
   const int cap_iters =
       Qc.count <= (int)gridDim.x && sweep_state[1] >= kSparseAfter ? sparse_iters : max_iters;
   const long long st = P.stride;
   long long my_iters = 0;
-  int idx = -1, end = 0;
-  while (Qc.next(idx, end)) {
-    const int ent = q.list[idx];
+  while (Qc.idx >= 0) {
+    const int ent = Qc.ent;
     const int t = ent & kEntryTile;
-    const TileRef r = tile_ref<S>(T, t);
+    const TileRef r = ONE ? tile_ref_of<S>(T.one, t) : tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
@@ -1464,6 +1560,8 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       for (int d = 0; d < S::KMAX; ++d) s_c[d][p] = 0;
     }
     __syncthreads();
+    // the next entry loads while the tile iterates
+    if (tid == 0) Qc.prefetch();
 
     int it = 0, any;
     do {
@@ -1530,6 +1628,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       any = block_any(pushed, s_w, par);
     } while (any && ++it < cap_iters);
     my_iters += it + (any ? 0 : 1);
+    if (tid == 0) Qc.publish();
 
     // ---- the flush's reads first: the flow that crossed into each ring cell
     // and the reverse-residual increments (LDS), and for increments into a
@@ -1621,6 +1720,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
     }
     if (fresh_tile && tid == 0) mat[t] = 1;
     if (!h_stored && tid == 0) hmat[t] = 1;
+    Qc.advance();
   }
   // one atomic per workgroup, spread over 64 counters
   if (tid == 0 && iter_total && my_iters)
@@ -1632,9 +1732,15 @@ void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
                  unsigned char *touched, unsigned char *defi, unsigned char *hmat,
                  int max_iters, int sparse_iters, const int *sweep_state,
                  long long *iter_total, hipStream_t st) {
-  JFC_DISPATCH(shape, (k_push<S><<<resident_grid<&k_push<S>>(nwg), 256, 0, st>>>(
-                          T, P, q, flag, mat, touched, defi, hmat, max_iters, sparse_iters,
-                          sweep_state, iter_total)));
+  if (T.ninst == 1) {
+    JFC_DISPATCH(shape, (k_push<S, true><<<resident_grid<&k_push<S, true>>(nwg), 256, 0, st>>>(
+                            T, P, q, flag, mat, touched, defi, hmat, max_iters, sparse_iters,
+                            sweep_state, iter_total)));
+  } else {
+    JFC_DISPATCH(shape, (k_push<S, false><<<resident_grid<&k_push<S, false>>(nwg), 256, 0, st>>>(
+                            T, P, q, flag, mat, touched, defi, hmat, max_iters, sparse_iters,
+                            sweep_state, iter_total)));
+  }
 }
 
 // ---- exact global relabel: tile-local Bellman-Ford to a fixpoint ------------------
