@@ -408,7 +408,52 @@ struct InstSums {
   }
 };
 
-static int row_grid(int nrows) { return nrows < 2048 ? (nrows > 0 ? nrows : 1) : 2048; }
+// Grid of a persistent (queue-consuming) launch: every workgroup must be
+// resident from the start -- one dispatched late finds the queue drained but
+// still runs its static first run after the others are done, stretching the
+// launch -- so the grid is capped at what the device holds at once for this
+// kernel (occupancy per CU, from its register and LDS use, times CUs).
+// Compute units of the current device, queried once per device (every
+// persistent launch sizes its grid with it).
+static int device_cus() {
+  static std::atomic<int> cache[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= 0 && dev < 16) {
+    const int c = cache[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  if (dev >= 0 && dev < 16) cache[dev].store(cus, std::memory_order_relaxed);
+  return cus;
+}
+
+template <auto Kernel, int Block = 256>
+static int resident_grid_b(int want) {
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, Kernel, Block, 0) != hipSuccess || n < 1)
+      n = 1;
+    return n;
+  }();
+  return std::max(1, std::min(want, per_cu * device_cus()));
+}
+
+template <auto Kernel>
+static int resident_grid(int want) {
+  return resident_grid_b<Kernel, 256>(want);
+}
+
+// Grid of a pass over the rows (nsteps row steps): one workgroup per step at
+// most, and no more than the device holds at once -- a second, partial round
+// of workgroups would run with most of the device idle.
+template <auto Kernel>
+static int row_grid(int nsteps) {
+  return resident_grid<Kernel>(std::max(1, nsteps));
+}
 
 // Fold terminal arcs: e = cs - ct; const += min(cs, ct);
 // deficit0 += max(ct - cs, 0); supply0 += max(cs - ct, 0).
@@ -505,13 +550,16 @@ void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
     launch_fill_u8(defi, T.ntiles, 0, st);
   }
   const int z = zero_h ? 1 : 0;
-  const int g = row_grid((T.nrows + T.row_step - 1) / T.row_step);
+  const int n = (T.nrows + T.row_step - 1) / T.row_step;
   if (masks == 2) {
-    JFC_DISPATCH(shape, (k_fold<S, 2><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+    JFC_DISPATCH(shape, (k_fold<S, 2><<<row_grid<&k_fold<S, 2>>(n), 256, 0, st>>>(
+                            T, P, red, active, defi, z, mask0)));
   } else if (masks == 1) {
-    JFC_DISPATCH(shape, (k_fold<S, 1><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+    JFC_DISPATCH(shape, (k_fold<S, 1><<<row_grid<&k_fold<S, 1>>(n), 256, 0, st>>>(
+                            T, P, red, active, defi, z, mask0)));
   } else {
-    JFC_DISPATCH(shape, (k_fold<S, 0><<<g, 256, 0, st>>>(T, P, red, active, defi, z, mask0)));
+    JFC_DISPATCH(shape, (k_fold<S, 0><<<row_grid<&k_fold<S, 0>>(n), 256, 0, st>>>(
+                            T, P, red, active, defi, z, mask0)));
   }
 }
 
@@ -568,8 +616,8 @@ void launch_cut_init(int shape, const Tiles &T, const Planes &P,
     launch_fill_i32((int *)def, 2LL * T.ninst, 0, st);
   }
   JFC_DISPATCH(shape,
-               (k_cut_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0, st>>>(
-                   T, P, seed, def)));
+               (k_cut_init<S><<<row_grid<&k_cut_init<S>>((T.nrows + T.row_step - 1) / T.row_step),
+                                256, 0, st>>>(T, P, seed, def)));
 }
 
 // graphcut_init_global_relabel (graphcut_global_relabel.cl:19-48) as a pass
@@ -613,8 +661,9 @@ __global__ __launch_bounds__(256) void k_gr_init(Tiles T, Planes P, int tag,
 void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigned char *defi,
                     hipStream_t st, bool cleared) {
   if (!cleared) launch_fill_u8(defi, T.ntiles, 0, st);
-  JFC_DISPATCH(shape, (k_gr_init<S><<<row_grid((T.nrows + T.row_step - 1) / T.row_step), 256, 0,
-                                       st>>>(T, P, tag, defi)));
+  const int n = (T.nrows + T.row_step - 1) / T.row_step;
+  JFC_DISPATCH(shape,
+               (k_gr_init<S><<<row_grid<&k_gr_init<S>>(n), 256, 0, st>>>(T, P, tag, defi)));
 }
 
 // Cut energy on the input capacities (QGCWidget.cpp:2452-2520), summed per
@@ -1189,66 +1238,15 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // serialise there.  Slot s of `ctr` holds the list count at [QSLOT*s] and
 // head h at [QSLOT*s + QWORD*(1+h)]; the consumer of slot s zeroes the other
 // slot's words for the next producer.
-struct QueueCursor {
-  int count, chunk, first, head_id, nheads;
-  int ahead;   // (lane 0) claim for the run after the current one
-  int *heads;
-  int *s_run;  // LDS word broadcasting the claimed run
-  __device__ __forceinline__ QueueCursor(const Queue &q, int *lds_word) {
-    if (blockIdx.x == 0 && threadIdx.x < 9) q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
-    count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
-    chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
-    heads = q.ctr + QSLOT * q.slot + QWORD;
-    s_run = lds_word;
-    const int b = blockIdx.x;
-    nheads = (gridDim.x % 8 == 0) ? 8 : 1;
-    head_id = b % nheads;
-    if (gridDim.x % 64 == 0) {
-      const int w = b >> 3;
-      first = (((w >> 3) << 3) + (b & 7)) * 8 + (w & 7);
-    } else {
-      first = b;
-    }
-  }
-  // Next list index for the whole workgroup (uniform); false once the list is
-  // drained.  Start with idx = -1, end = 0.  The claim for the following run
-  // is issued when a run starts and read when it ends, so its latency hides
-  // behind the run's tiles; every claimed run is processed at the next call
-  // or lies past the end.  Between two calls the caller passes at least one
-  // barrier, so s_run is not overwritten while read.
-  __device__ __forceinline__ bool next(int &idx, int &end) {
-    if (idx + 1 < end) {
-      ++idx;
-      return true;
-    }
-    int run = first;
-    if (idx >= 0) {
-      if (threadIdx.x == 0) *s_run = ahead;
-      __syncthreads();
-      const int k = *s_run;  // k-th run handed out by this head (< 0: none)
-      if (k < 0) {
-        idx = end = count;
-        return false;
-      }
-      run = (int)gridDim.x + 8 * (head_id + nheads * (k >> 3)) + (k & 7);
-    }
-    idx = run * chunk;
-    end = min(idx + chunk, count);
-    const bool ok = idx < count;
-    if (ok && threadIdx.x == 0)
-      ahead = (long long)gridDim.x * chunk < count ? atomicAdd(&heads[QWORD * head_id], 1) : -1;
-    return ok;
-  }
-};
-
-// The cursor of k_push: QueueCursor's runs and claims, but a workgroup learns
-// its next list entry while it discharges the current tile, so that a visit
-// starts with neither a global load nor a barrier of its own.  Thread 0 works
-// out the next position during the local iterations (the rest of the current
-// run, else the run its claim obtained, in which case it claims the run after
-// that at once) and loads its entry; it publishes position and entry in LDS
-// before the visit's last barrier, after which every thread reads them.
-struct PushCursor {
+// The cursor of the tile-visit kernels (push, relax, cut flood).  A
+// workgroup learns its next list entry while it discharges the current tile,
+// so that a visit starts with neither a global load nor a barrier of its own:
+// thread 0 works out the next position during the local iterations (the rest
+// of the current run, else the run its claim obtained, in which case it
+// claims the run after that at once) and loads its entry; it publishes
+// position and entry in LDS before the visit's last barrier, after which
+// every thread reads them.
+struct VisitCursor {
   int count, chunk, head_id, nheads;
   int *heads;
   const int *list;
@@ -1256,7 +1254,7 @@ struct PushCursor {
   int nidx, nend, nent;  // (thread 0) the next position, its run's end, its entry
   int *s;                // LDS: next position, run end, entry
   int idx, end, ent;     // current position (uniform; idx < 0: drained)
-  __device__ __forceinline__ PushCursor(const Queue &q, int *lds3) {
+  __device__ __forceinline__ VisitCursor(const Queue &q, int *lds3) {
     if (blockIdx.x == 0 && threadIdx.x < 9) q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
     count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
     chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
@@ -1315,45 +1313,6 @@ struct PushCursor {
     ent = __builtin_amdgcn_readfirstlane(s[2]);
   }
 };
-
-// Grid of a persistent (queue-consuming) launch: every workgroup must be
-// resident from the start -- one dispatched late finds the queue drained but
-// still runs its static first run after the others are done, stretching the
-// launch -- so the grid is capped at what the device holds at once for this
-// kernel (occupancy per CU, from its register and LDS use, times CUs).
-// Compute units of the current device, queried once per device (every
-// persistent launch sizes its grid with it).
-static int device_cus() {
-  static std::atomic<int> cache[16];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-  if (dev >= 0 && dev < 16) {
-    const int c = cache[dev].load(std::memory_order_relaxed);
-    if (c > 0) return c;
-  }
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus < 1)
-    cus = 1;
-  if (dev >= 0 && dev < 16) cache[dev].store(cus, std::memory_order_relaxed);
-  return cus;
-}
-
-template <auto Kernel, int Block = 256>
-static int resident_grid_b(int want) {
-  static const int per_cu = [] {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, Kernel, Block, 0) != hipSuccess || n < 1)
-      n = 1;
-    return n;
-  }();
-  return std::max(1, std::min(want, per_cu * device_cus()));
-}
-
-template <auto Kernel>
-static int resident_grid(int want) {
-  return resident_grid_b<Kernel, 256>(want);
-}
 
 // ---- residual-arc masks and lazy residual planes ----------------------------------
 //
@@ -1454,7 +1413,7 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
   __shared__ int s_next[3];
   __shared__ int s_w[8];
   int par = 0;
-  PushCursor Qc(q, s_next);
+  VisitCursor Qc(q, s_next);
   // the sparse cap from the solve's kSparseAfter-th sweep on: a solve that
   // ends within a few sweeps has nothing to gain from long visits
   const int cap_iters =
@@ -1749,7 +1708,7 @@ void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
 // LDS until the tile is stable; tiles whose halo changed are revisited.  The
 // fixpoint over all tiles is the exact breadth-first distance to the nearest
 // deficit node (graphcut_global_relabel.cl:50-124).
-template <class S, bool FIRST>
+template <class S, bool FIRST, bool ONE>
 __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty,
                                                unsigned char *active, int tag) {
@@ -1757,14 +1716,13 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
   __shared__ int s_h[S::PADN];
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  __shared__ int s_run;
+  __shared__ int s_next[3];
   __shared__ int s_w[8];
   int par = 0;
-  QueueCursor Qc(q, &s_run);
-  int idx = -1, end = 0;
-  while (Qc.next(idx, end)) {
-    const int t = q.list[idx];
-    const TileRef r = tile_ref<S>(T, t);
+  VisitCursor Qc(q, s_next);
+  while (Qc.idx >= 0) {
+    const int t = Qc.ent;
+    const TileRef r = ONE ? tile_ref_of<S>(T.one, t) : tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
@@ -1813,6 +1771,8 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
       s_h[hp[c]] = v;
     }
     __syncthreads();
+    // the next entry loads while the tile iterates
+    if (tid == 0) Qc.prefetch();
     int any;
     do {
       int ch = 0;
@@ -1856,8 +1816,10 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     }
     // graphcut_count_node fused in: the tile's last visit leaves its
     // active flag (a node with excess and a finite height)
+    if (tid == 0) Qc.publish();
     const int act = block_any(valid && e0 > 0 && s_h[L.me] < HINF, s_w, par);
     if (tid == 0) active[t] = (unsigned char)act;
+    Qc.advance();
   }
 }
 
@@ -1875,7 +1837,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 
-template <class S, bool FIRST>
+template <class S, bool FIRST, bool ONE>
 __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
                                                 unsigned char *dirty, unsigned char *active,
                                                 int tag) {
@@ -1883,13 +1845,12 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
   constexpr int NJ = S::NT / 64;
   constexpr int NR = (Ring<S>::N + 63) / 64;
   __shared__ int s_h[S::PADN];
-  __shared__ int s_run;
+  __shared__ int s_next[3];
   const int lane = threadIdx.x;
-  QueueCursor Qc(q, &s_run);
-  int idx = -1, end = 0;
-  while (Qc.next(idx, end)) {
-    const int t = q.list[idx];
-    const TileRef r = tile_ref<S>(T, t);
+  VisitCursor Qc(q, s_next);
+  while (Qc.idx >= 0) {
+    const int t = Qc.ent;
+    const TileRef r = ONE ? tile_ref_of<S>(T.one, t) : tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = I.K;
     // loads: ring heights (device-coherent and tagged, as in k_relax) and
@@ -1947,6 +1908,8 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
       s_h[Ring<S>::pos(k)] = v;
     }
     wave_lds_sync();
+    // the next entry loads while the tile iterates
+    if (lane == 0) Qc.prefetch();
     bool any;
     do {
       bool ch = false;
@@ -1995,8 +1958,12 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
       }
     }
     const bool a = __ballot(act) != 0;
-    if (lane == 0) active[t] = (unsigned char)a;
+    if (lane == 0) {
+      active[t] = (unsigned char)a;
+      Qc.publish();
+    }
     wave_lds_sync();
+    Qc.advance();
   }
 }
 
@@ -2004,25 +1971,29 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
 // 3D tiles iterate more per visit (fronts cross 8x8x4 tiles in z as well)
 // and there the 256-lane visit's four parallel waves win (config 4's
 // relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
+template <bool FIRST, bool ONE>
+static void relax_launch(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
+                         unsigned char *dirty, unsigned char *active, int tag, hipStream_t st) {
+  if (shape_is_3d(shape)) {
+    JFC_DISPATCH(shape, (k_relax<S, FIRST, ONE><<<resident_grid<&k_relax<S, FIRST, ONE>>(nwg),
+                                                  256, 0, st>>>(T, P, q, dirty, active, tag)));
+  } else {
+    JFC_DISPATCH(shape,
+                 (k_relax_w<S, FIRST, ONE><<<resident_grid_b<&k_relax_w<S, FIRST, ONE>, 64>(
+                                                 4 * nwg),
+                                             64, 0, st>>>(T, P, q, dirty, active, tag)));
+  }
+}
+
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
                   bool first_pass, int tag, hipStream_t st) {
-  if (shape_is_3d(shape)) {
-    if (first_pass) {
-      JFC_DISPATCH(shape, (k_relax<S, true><<<resident_grid<&k_relax<S, true>>(nwg), 256, 0, st>>>(
-                              T, P, q, dirty, active, tag)));
-    } else {
-      JFC_DISPATCH(shape, (k_relax<S, false><<<resident_grid<&k_relax<S, false>>(nwg), 256, 0,
-                                                st>>>(T, P, q, dirty, active, tag)));
-    }
-    return;
-  }
   if (first_pass) {
-    JFC_DISPATCH(shape, (k_relax_w<S, true><<<resident_grid_b<&k_relax_w<S, true>, 64>(4 * nwg),
-                                              64, 0, st>>>(T, P, q, dirty, active, tag)));
+    if (T.ninst == 1) relax_launch<true, true>(shape, nwg, T, P, q, dirty, active, tag, st);
+    else relax_launch<true, false>(shape, nwg, T, P, q, dirty, active, tag, st);
   } else {
-    JFC_DISPATCH(shape, (k_relax_w<S, false><<<resident_grid_b<&k_relax_w<S, false>, 64>(4 * nwg),
-                                               64, 0, st>>>(T, P, q, dirty, active, tag)));
+    if (T.ninst == 1) relax_launch<false, true>(shape, nwg, T, P, q, dirty, active, tag, st);
+    else relax_launch<false, false>(shape, nwg, T, P, q, dirty, active, tag, st);
   }
 }
 
@@ -2039,22 +2010,21 @@ void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
 // :142-242): P.lab is the tag plane, a tagged node tags its axis neighbours
 // (4 in 2D, 6 in 3D) that are on the source side of the cut (`object`).
 // LDS codes: 0 open, 1 labelled on entry, 2 labelled here, 4 closed.
-template <class S, bool TAG>
+template <class S, bool TAG, bool ONE>
 __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty, int *cert,
                                                const unsigned char *object) {
   __shared__ unsigned char s_l[S::PADN];
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
-  __shared__ int s_run;
+  __shared__ int s_next[3];
   __shared__ int s_w[8];
   int par = 0;
-  QueueCursor Qc(q, &s_run);
+  VisitCursor Qc(q, s_next);
   constexpr unsigned kAxis = S::D3 ? 0x3fu : 0xfu;
-  int idx = -1, end = 0;
-  while (Qc.next(idx, end)) {
-    const int t = q.list[idx];
-    const TileRef r = tile_ref<S>(T, t);
+  while (Qc.idx >= 0) {
+    const int t = Qc.ent;
+    const TileRef r = ONE ? tile_ref_of<S>(T.one, t) : tile_ref<S>(T, t);
     const InstDesc &I = r.I;
     const int K = TAG ? (S::D3 ? 6 : 4) : I.K;
     const int gx = r.ox + L.lx, gy = r.oy + L.ly, gz = r.oz + L.lz;
@@ -2100,6 +2070,8 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
       s_l[hp[c]] = lv;
     }
     __syncthreads();
+    // the next entry loads while the tile iterates
+    if (tid == 0) Qc.prefetch();
     int any;
     do {
       int ch = 0;
@@ -2134,22 +2106,32 @@ __global__ __launch_bounds__(256) void k_reach(Tiles T, Planes P, Queue q,
       P.lab[hc[c].g] = 1;
       dirty[tile_of<S>(I, X, Y, Z)] = 1;
     }
+    if (tid == 0) Qc.publish();
     __syncthreads();
+    Qc.advance();
   }
+}
+
+template <bool TAG, bool ONE>
+static void reach_launch(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
+                         unsigned char *dirty, int *cert, const unsigned char *object,
+                         hipStream_t st) {
+  JFC_DISPATCH(shape, (k_reach<S, TAG, ONE><<<resident_grid<&k_reach<S, TAG, ONE>>(nwg), 256, 0,
+                                              st>>>(T, P, q, dirty, cert, object)));
 }
 
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st) {
-  JFC_DISPATCH(shape, (k_reach<S, false><<<resident_grid<&k_reach<S, false>>(nwg), 256, 0, st>>>(
-                          T, P, q, dirty, cert, nullptr)));
+  if (T.ninst == 1) reach_launch<false, true>(shape, nwg, T, P, q, dirty, cert, nullptr, st);
+  else reach_launch<false, false>(shape, nwg, T, P, q, dirty, cert, nullptr, st);
 }
 
 void launch_tag(int shape, int nwg, const Tiles &T, const Planes &P,
                 const Queue &q, unsigned char *dirty, const unsigned char *object,
                 hipStream_t st) {
-  JFC_DISPATCH(shape, (k_reach<S, true><<<resident_grid<&k_reach<S, true>>(nwg), 256, 0, st>>>(
-                          T, P, q, dirty, nullptr, object)));
+  if (T.ninst == 1) reach_launch<true, true>(shape, nwg, T, P, q, dirty, nullptr, object, st);
+  else reach_launch<true, false>(shape, nwg, T, P, q, dirty, nullptr, object, st);
 }
 
 }  // namespace jfc
