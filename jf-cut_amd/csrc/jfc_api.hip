@@ -1151,7 +1151,7 @@ struct Solver {
       Queue q = queue_from_list();
       const int n = g->color_off[c + 1] - g->color_off[c];
       launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, g->d_touched,
-                     q, g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_);
+                     q, g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_, g->d_hmat);
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
                   g->d_touched, g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,

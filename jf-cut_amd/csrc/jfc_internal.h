@@ -195,14 +195,18 @@ void launch_sweep_resume(const SweepCtl &c, hipStream_t st);
 
 // Queue list entries: tile index, plus, when compacted with `mat`, kEntryMat
 // if mat[t] is set, else kEntryClean if touched[t] is clear (no visit and no
-// flush has written the tile's planes or mask words in this solve).
+// flush has written the tile's planes or mask words in this solve); when
+// compacted with `hmat`, kEntryHeights if hmat[t] is set (the tile's heights
+// are stored in this solve).
 constexpr int kEntryMat = 1 << 30;
 constexpr int kEntryClean = 1 << 29;
-constexpr int kEntryTile = kEntryClean - 1;
+constexpr int kEntryHeights = 1 << 28;
+constexpr int kEntryTile = kEntryHeights - 1;
 // stop (or null): a nonzero word makes the compaction a no-op.
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const unsigned char *touched, const Queue &q,
-                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st);
+                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st,
+                    const unsigned char *hmat = nullptr);
 // mat: per tile, 1 once every residual plane of the tile was written in
 // this solve (cleared when a solve starts).  defi: a visit rewrites its
 // tile's deficit flag (pushes never create deficits, so unvisited tiles'

@@ -1090,7 +1090,8 @@ constexpr int kCompactBlock = 1024;
 __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n, int reverse,
                                                            unsigned char *flag,
                                                            const unsigned char *mat,
-                                                           const unsigned char *touched, Queue q,
+                                                           const unsigned char *touched,
+                                                           const unsigned char *hmat, Queue q,
                                                            int *sweep_cnt, long long *total,
                                                            const int *stop) {
   __shared__ int s_cnt[kCompactBlock / 64];
@@ -1128,6 +1129,7 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(const int *src, int n
     for (int k = 0; k < w; ++k) base += s_cnt[k];
     int ent = t;
     if (mat) ent |= mat[t] ? kEntryMat : (touched && !touched[t] ? kEntryClean : 0);
+    if (hmat && hmat[t]) ent |= kEntryHeights;
     q.list[base + __popcll(m & ((1ull << lane) - 1ull))] = ent;
   }
 }
@@ -1212,10 +1214,11 @@ void launch_sweep_resume(const SweepCtl &c, hipStream_t st) {
 
 void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
                     const unsigned char *mat, const unsigned char *touched, const Queue &q,
-                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st) {
+                    int *sweep_cnt, long long *total, const int *stop, hipStream_t st,
+                    const unsigned char *hmat) {
   if (n <= 0) return;
   k_compact<<<(n + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, st>>>(
-      src, n, reverse, flag, mat, touched, q, sweep_cnt, total, stop);
+      src, n, reverse, flag, mat, touched, hmat, q, sweep_cnt, total, stop);
 }
 
 // ---- persistent tile queue ----------------------------------------------------------
@@ -1476,8 +1479,9 @@ __global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
       hm[c] = P.rmask[hc[c].g];
     }
     // heights not yet stored in this solve read as 0 (the start labelling);
-    // the visit then stores all of them
-    const bool h_stored = hmat[t] != 0;
+    // the visit then stores all of them.  The compaction read hmat[t] into
+    // the entry, so no load stands between the entry and the plane loads.
+    const bool h_stored = (ent & kEntryHeights) != 0;
     const int hw0 = P.h[gv];
     int e0 = P.e[gv], h0 = h_stored ? hw0 & HMASK : 0;
     unsigned rm = 0;
