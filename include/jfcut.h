@@ -296,9 +296,9 @@ int jfc_graph_size(const jfc_graph *g, int64_t *nodes, int32_t *kmax);
 /* Time `reps` solves of the graph (after one warm-up) with every tile shape of
  * its dimensionality (2D: 32x8, 64x4, 16x16; 3D: 8x8x4, 32x4x2), crossed with
  * every local-iteration cap of {2, 3, 4} unless opt->max_local_iters fixes
- * it and with initial_relabel 0 and 1 unless opt->initial_relabel fixes it;
- * then, on the fastest of those, stall_certify 1 unless opt->stall_certify
- * fixes it, and then sparse_local_iters -1, 8 and 64 unless
+ * it, with initial_relabel 0 and 1 unless opt->initial_relabel fixes it and
+ * with stall_certify 0 and 1 unless opt->stall_certify fixes it; then, on
+ * the fastest of those, sparse_local_iters -1, 8 and 64 unless
  * opt->sparse_local_iters fixes it; all under the other options of opt (may
  * be NULL).  The fastest choice is what the graph's solves use when those
  * options are left at their defaults (JFC_TILE_DEFAULT, 0, -1, -1, 0).  The

@@ -1618,12 +1618,15 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   // the push sweeps settle most of the flow first (config 2: 4.8 -> 8.5 ms)
   const int ir_fixed = o.initial_relabel >= 0 ? o.initial_relabel : -1;
   const int ni = ir_fixed >= 0 ? 1 : 2;
-  // then, one at a time on the best of those: whether a stalled active set
-  // first tries the certificate (it pays where the last relabel would only
-  // prove the remaining excess cut off), and the cap of sparse launches
-  // (large where flow must cross whole tiles, small where visits of a few
-  // tiles would iterate without effect)
+  // and whether a stalled active set first tries the certificate (it pays
+  // where the last relabel would only prove the remaining excess cut off):
+  // it changes how many sweeps and relabels a shape needs, so it is tried
+  // with every shape rather than after the shape is chosen.  Then, on the
+  // best of those, the cap of sparse launches (large where flow must cross
+  // whole tiles, small where visits of a few tiles would iterate without
+  // effect).
   const int sc_fixed = o.stall_certify >= 0 ? o.stall_certify : -1;
+  const int ns = sc_fixed >= 0 ? 1 : 2;
   const int sp_fixed = o.sparse_local_iters != 0 ? o.sparse_local_iters : 0;
   auto time_it = [&](const jfc_options &t, double *med) -> int {
     std::vector<double> ms;
@@ -1650,21 +1653,17 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
     }
     return JFC_OK;
   };
-  o.stall_certify = sc_fixed >= 0 ? sc_fixed : 0;
   o.sparse_local_iters = sp_fixed ? sp_fixed : kSparseIters;
   for (int c = 0; c < nc; ++c)
     for (int m = 0; m < nm; ++m)
-      for (int i = 0; i < ni; ++i) {
-        o.tile_shape = cand[c];
-        o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
-        o.initial_relabel = ir_fixed >= 0 ? ir_fixed : i;
-        if (int rc = consider(o)) return rc;
-      }
-  if (sc_fixed < 0) {
-    jfc_options t = best_o;
-    t.stall_certify = 1;
-    if (int rc = consider(t)) return rc;
-  }
+      for (int i = 0; i < ni; ++i)
+        for (int k = 0; k < ns; ++k) {
+          o.tile_shape = cand[c];
+          o.max_local_iters = mli_fixed ? mli_fixed : mcand[m];
+          o.initial_relabel = ir_fixed >= 0 ? ir_fixed : i;
+          o.stall_certify = sc_fixed >= 0 ? sc_fixed : k;
+          if (int rc = consider(o)) return rc;
+        }
   if (!sp_fixed) {
     const int spc[3] = {-1, 8, 64};
     const jfc_options base = best_o;
