@@ -112,6 +112,10 @@ int log2_round(double v) { return v <= 1.0 ? 0 : (int)std::lround(std::log2(v));
 // iterations of the push kernel (workgroup b adds to slot b % 64, so no
 // single address takes one atomic per workgroup).
 constexpr int kIterSlots = 5, kTotals = kIterSlots + 64;
+// Graphs of at most this many tiles: a search pass gives every tile a
+// workgroup of its own (a device of 256 compute units keeps several tile
+// visits per unit resident), so its passes skip the distance ordering.
+constexpr int kSmallTiles = 1024;
 // words of the results block besides the per-instance ones: the counters,
 // the sweep state (kSweepState ints) and 4 flag words
 constexpr size_t kResFixed = kTotals + (kSweepState + 4) / 2;
@@ -1005,10 +1009,18 @@ struct Solver {
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
     // relabel seeds: the tiles holding a deficit node (d_defi, kept by the
-    // fold and the push visits); cut and tag seeds: d_seed from their init
-    launch_tile_order(g->shape, T, mode == kRelabel ? g->d_defi : g->d_seed, g->d_jf0, g->d_jf1,
-                      g->d_tdist, g->d_hist, g->nbins, g->d_order, g->max_tile_dim, nullptr,
-                      g->lines(), o.tile_order, st, cleared);
+    // fold and the push visits); cut and tag seeds: d_seed from their init.
+    // A graph of at most kSmallTiles tiles gives every tile a workgroup of
+    // its own in each pass, so the order is moot: its passes take the tiles
+    // in colour order and skip the five launches of the distance ordering.
+    int *order = g->d_order;
+    if (g->ntiles <= kSmallTiles) {
+      order = g->d_color_tiles;
+    } else {
+      launch_tile_order(g->shape, T, mode == kRelabel ? g->d_defi : g->d_seed, g->d_jf0,
+                        g->d_jf1, g->d_tdist, g->d_hist, g->nbins, g->d_order, g->max_tile_dim,
+                        nullptr, g->lines(), o.tile_order, st, cleared);
+    }
     if (!cleared) launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
     const int nwg = std::min(g->nwg, g->ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
@@ -1024,11 +1036,11 @@ struct Solver {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
-      launch_compact(g->d_order, g->ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
+      launch_compact(order, g->ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
                      nullptr, st);
       flood(q, false);
     } else {
-      flood(Queue{g->d_order, g->d_ctr, g->slot, g->ntiles}, true);
+      flood(Queue{order, g->d_ctr, g->slot, g->ntiles}, true);
     }
     advance();
     *passes += 1;
@@ -1042,7 +1054,7 @@ struct Solver {
     auto issue_group = [&](int k) -> int {
       for (int j = 0; j < kPassGroup; ++j, ++pass) {
         Queue qc = queue_from_list();
-        launch_compact(g->d_order, g->ntiles, pass & 1, g->d_dirty, nullptr, nullptr, qc, nullptr,
+        launch_compact(order, g->ntiles, pass & 1, g->d_dirty, nullptr, nullptr, qc, nullptr,
                        total, nullptr, st);
         flood(qc, false);
         advance();
@@ -1150,8 +1162,22 @@ struct Solver {
     for (int c = 0; c < g->ncolors; ++c) {
       Queue q = queue_from_list();
       const int n = g->color_off[c + 1] - g->color_off[c];
-      launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat, g->d_touched,
-                     q, g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_, g->d_hmat);
+      if (n <= kSmallTiles) {
+        // a colour class of a small graph: the push workgroups pick their
+        // flagged tiles from the class's list themselves (one per workgroup)
+        q = Queue{g->d_color_tiles + g->color_off[c], g->d_ctr, g->slot, n};
+        q.pick = g->d_flag;
+        q.mat = g->d_mat;
+        q.touched = g->d_touched;
+        q.hmat = g->d_hmat;
+        q.cnt = g->d_sweep_state;
+        q.total = g->d_total;
+        q.stop = g->d_sweep_state + 2;
+      } else {
+        launch_compact(g->d_color_tiles + g->color_off[c], n, 0, g->d_flag, g->d_mat,
+                       g->d_touched, q, g->d_sweep_state, g->d_total, g->d_sweep_state + 2, s_,
+                       g->d_hmat);
+      }
       hipEvent_t a = (timed && o.profile) ? tm.mark() : nullptr;
       launch_push(g->shape, std::min(g->nwg, std::max(1, n)), T, P, q, g->d_flag, g->d_mat,
                   g->d_touched, g->d_defi, g->d_hmat, o.max_local_iters, o.sparse_local_iters,

@@ -105,6 +105,17 @@ struct Queue {
   int *ctr;
   int slot;
   int fixed_count;  // >= 0: consume list[0..fixed_count) instead of ctr count
+  // Direct list (pick set): list[0..fixed_count) holds every tile of the
+  // launch unfiltered, and the consumer does the compaction's work itself --
+  // a tile is visited only if pick[t] is set (and pick[t] is cleared), its
+  // entry bits come from mat / touched / hmat, and each visit adds 1 to
+  // *cnt and *total; nothing is visited once *stop is nonzero.  For lists
+  // no longer than the launch's grid, it saves the compaction launch.
+  unsigned char *pick = nullptr;
+  const unsigned char *mat = nullptr, *touched = nullptr, *hmat = nullptr;
+  int *cnt = nullptr;
+  long long *total = nullptr;
+  const int *stop = nullptr;
 };
 
 // Sets the thread-local message returned by jfc_last_error().

@@ -1249,7 +1249,11 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // claims the run after that at once) and loads its entry; it publishes
 // position and entry in LDS before the visit's last barrier, after which
 // every thread reads them.
+// A direct list (Queue::pick) is walked in strides of the grid, workgroup b
+// from position b, each workgroup taking the next position whose tile is
+// picked.
 struct VisitCursor {
+  Queue q;
   int count, chunk, head_id, nheads;
   int *heads;
   const int *list;
@@ -1257,7 +1261,7 @@ struct VisitCursor {
   int nidx, nend, nent;  // (thread 0) the next position, its run's end, its entry
   int *s;                // LDS: next position, run end, entry
   int idx, end, ent;     // current position (uniform; idx < 0: drained)
-  __device__ __forceinline__ VisitCursor(const Queue &q, int *lds3) {
+  __device__ __forceinline__ VisitCursor(const Queue &q_, int *lds3) : q(q_) {
     if (blockIdx.x == 0 && threadIdx.x < 9) q.ctr[QSLOT * (q.slot ^ 1) + QWORD * threadIdx.x] = 0;
     count = q.fixed_count >= 0 ? q.fixed_count : q.ctr[QSLOT * q.slot];
     chunk = min(16, max(1, count / (int)(gridDim.x * 4)));
@@ -1265,6 +1269,17 @@ struct VisitCursor {
     list = q.list;
     s = lds3;
     const int b = blockIdx.x;
+    if (q.pick) {
+      chunk = 1;
+      ahead = -1;
+      if (threadIdx.x == 0) {
+        find(b);
+        publish();
+      }
+      __syncthreads();
+      advance();
+      return;
+    }
     nheads = (gridDim.x % 8 == 0) ? 8 : 1;
     head_id = b % nheads;
     int first = b;
@@ -1286,8 +1301,41 @@ struct VisitCursor {
       ent = 0;
     }
   }
+  // thread 0, direct list: take position i's tile if it is picked (and the
+  // solve has not stopped sweeping); its entry, or -1
+  __device__ __forceinline__ int take(int i) const {
+    const int t = list[i];
+    if (!q.pick[t]) return -1;
+    q.pick[t] = 0;
+    int e = t;
+    if (q.mat) e |= q.mat[t] ? kEntryMat : (q.touched && !q.touched[t] ? kEntryClean : 0);
+    if (q.hmat && q.hmat[t]) e |= kEntryHeights;
+    if (q.cnt) atomicAdd(q.cnt, 1);
+    if (q.total) atomicAdd((unsigned long long *)q.total, 1ull);
+    return e;
+  }
+  // thread 0, direct list: the first taken position from i on, in strides of
+  // the grid
+  __device__ __forceinline__ void find(int i) {
+    nidx = -1;
+    nend = nent = 0;
+    if (q.stop && *q.stop) return;
+    for (; i < count; i += (int)gridDim.x) {
+      const int e = take(i);
+      if (e >= 0) {
+        nidx = i;
+        nend = i + 1;
+        nent = e;
+        return;
+      }
+    }
+  }
   // thread 0, during a visit: the position after the current one, and its entry
   __device__ __forceinline__ void prefetch() {
+    if (q.pick) {
+      find(idx + (int)gridDim.x);
+      return;
+    }
     nidx = -1;
     if (idx + 1 < end) {
       nidx = idx + 1;
@@ -1401,13 +1449,11 @@ constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 template <class S, bool ONE>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
 // without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
-// 8, where it spills).
-__global__ __launch_bounds__(256, 7) void k_push(Tiles T, Planes P, Queue q,
-                                              unsigned char *flag, unsigned char *mat,
-                                              unsigned char *touched, unsigned char *defi,
-                                              unsigned char *hmat, int max_iters,
-                                              int sparse_iters, const int *sweep_state,
-                                              long long *iter_total) {
+// 8, where it spills).  Two ring cells per lane (32x4x2) take ~125: 4 waves.
+__global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
+    Tiles T, Planes P, Queue q, unsigned char *flag, unsigned char *mat, unsigned char *touched,
+    unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
+    const int *sweep_state, long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
