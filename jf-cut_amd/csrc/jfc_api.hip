@@ -1058,7 +1058,10 @@ struct Solver {
     // after convergence compacts empty lists and changes nothing.
     int pass = 1;
     auto issue_group = [&](int k) -> int {
-      for (int j = 0; j < kPassGroup; ++j, ++pass) {
+      // the first group is short: a cut flood has done its work within a
+      // pass or two, and passes queued past the end each cost two launches
+      const int size = k == 0 ? 2 : kPassGroup;
+      for (int j = 0; j < size; ++j, ++pass) {
         Queue qc = queue_from_list();
         launch_compact(order, g->ntiles, pass & 1, g->d_dirty, nullptr, nullptr, qc, nullptr,
                        total, nullptr, st);
