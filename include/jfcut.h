@@ -95,11 +95,14 @@ typedef struct {
                              push per colour class) as one captured hipGraph
                              (default); 0 = launch them one by one.  profile = 1
                              always launches them one by one. */
-  int tile_order;         /* tile distance field that orders relabel and cut
-                             passes: 0 = separable taxicab distance transform
-                             (default, D launches), 1 = jump flooding
+  int tile_order;         /* tile distance field that orders the relabel
+                             passes of graphs of more than 1024 tiles: 0 =
+                             separable taxicab distance transform (default, D
+                             launches), 1 = jump flooding
                              (graphcut_bfs_block.cl:43-63, 2 ceil(log2) + 2
-                             launches); the distances are identical */
+                             launches); the distances are identical.  Cut
+                             and tag passes, and every pass of a smaller
+                             graph, take the tiles in colour order */
   int method;             /* JFC_METHOD_JFCUT (0): tile push-relabel in LDS;
                              JFC_METHOD_FASTCUT (1): the widget's Fast-Cut
                              comparator (radioButtonCutMethodJFCut unchecked,
