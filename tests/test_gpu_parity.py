@@ -64,6 +64,27 @@ def test_random_grids_gpu(gpu, conn, shape):
         g.close()
 
 
+# 32x8 tiles: 1024 (small-graph paths: push launches pick their own tiles,
+# search passes in colour order), 1056 (colour classes still small enough for
+# self-picking pushes, relabel passes in distance order) and 8250 (compacted
+# push lists too)
+@pytest.mark.parametrize("conn,nx,ny", [(4, 1024, 256), (4, 1056, 256), (8, 1056, 256),
+                                        (8, 2100, 1000)])
+@pytest.mark.parametrize("opts", [dict(), dict(initial_relabel=1),
+                                  dict(initial_relabel=1, stall_certify=1, tile_order=1)])
+def test_small_and_large_launch_paths_gpu(gpu, conn, nx, ny, opts):
+    rng = np.random.default_rng(nx + ny + conn)
+    cs, ct, caps = random_grid(rng, nx, ny, 1, conn)
+    f, l, ok = ol.maxflow(nx, ny, 1, conn, cs, ct, caps)
+    say("case launch paths", conn, nx, ny, opts)
+    g, st = solve_grid(gpu, nx, ny, 1, conn, cs, ct, caps, tile_shape="32x8", **opts)
+    assert st.converged == 1
+    assert g.flow() == f
+    assert np.array_equal(g.labels(), l)
+    assert g.energy() == f
+    g.close()
+
+
 @pytest.mark.parametrize("opts", [dict(tile_shape="64x4"), dict(tile_shape="16x16"),
                                   dict(max_local_iters=1), dict(relabel_work=0.0),
                                   dict(check_every=1, max_local_iters=1000),
