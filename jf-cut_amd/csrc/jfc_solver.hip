@@ -1244,17 +1244,17 @@ void launch_compact(const int *src, int n, int reverse, unsigned char *flag,
 // serialise there.  Slot s of `ctr` holds the list count at [QSLOT*s] and
 // head h at [QSLOT*s + QWORD*(1+h)]; the consumer of slot s zeroes the other
 // slot's words for the next producer.
-// The cursor of the tile-visit kernels (push, relax, cut flood).  A
-// workgroup learns its next list entry while it discharges the current tile,
-// so that a visit starts with neither a global load nor a barrier of its own:
-// thread 0 works out the next position during the local iterations (the rest
-// of the current run, else the run its claim obtained, in which case it
-// claims the run after that at once) and loads its entry; it publishes
-// position and entry in LDS before the visit's last barrier, after which
-// every thread reads them.
-// A direct list (Queue::pick) is walked in strides of the grid, workgroup b
-// from position b, each workgroup taking the next position whose tile is
-// picked.
+//
+// VisitCursor walks a queue that way for the tile-visit kernels (push, relax,
+// cut flood).  A workgroup learns its next list entry while it discharges
+// the current tile, so that a visit starts with neither a global load nor a
+// barrier of its own: thread 0 works out the next position during the local
+// iterations (the rest of the current run, else the run its claim obtained,
+// in which case it claims the run after that at once) and loads its entry;
+// it publishes position and entry in LDS before the visit's last barrier,
+// after which every thread reads them.  A direct list (Queue::pick) is
+// walked in strides of the grid instead, workgroup b from position b, each
+// workgroup taking the next position whose tile is picked.
 struct VisitCursor {
   Queue q;
   int count, chunk, head_id, nheads;
