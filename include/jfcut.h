@@ -132,6 +132,8 @@ typedef struct {
 #define JFC_TILE_64x4 2
 #define JFC_TILE_16x16 3
 #define JFC_TILE_8x8x4 4
+/* 32x4x2 push tiles (whole 128-byte rows) run their global relabels on
+ * 8x8x4 tiles, whose relabel fronts cross half as many tiles in y and z. */
 #define JFC_TILE_32x4x2 5
 
 typedef struct {

@@ -170,6 +170,24 @@ struct jfc_graph {
   unsigned char *d_touched = nullptr;  // per tile: planes or masks written by a flush
   int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
       *d_order = nullptr;
+  // The tiling of the 3D global relabel when it is not the push tiling:
+  // 32x4x2 push tiles read whole 128-byte rows (an 8x8x4 tile's rows are 32
+  // bytes), but a relabel front crosses about one tile per pass, and 32x4x2
+  // tiles are half as deep in y and z, so their relabels run on 8x8x4 tiles
+  // (config 3: push 1.48 ms on 32x4x2 tiles against 2.81 on 8x8x4, relabel
+  // 2.95 ms on 8x8x4 against 4.56 on 32x4x2).  shape 0: the relabel runs on
+  // the push tiling.  The relabel writes the push tiling's active flags.
+  struct RelabelTiles {
+    int shape = 0;
+    std::vector<InstDesc> inst;
+    InstDesc *d_inst = nullptr;
+    int *d_tile_inst = nullptr;
+    int ntiles = 0, max_tile_dim = 1, nbins = 2;
+    unsigned char *d_dirty = nullptr, *d_defi = nullptr;
+    int *d_jf0 = nullptr, *d_jf1 = nullptr, *d_tdist = nullptr, *d_hist = nullptr,
+        *d_order = nullptr, *d_lines = nullptr;
+    int lines_total[3] = {0, 0, 0};
+  } rt;
   // One device block holds every word a solve reads back at its end (one
   // copy): the per-instance sums, the remaining deficits, the counters, the
   // sweep state and the flag words, in this order (res_words long longs).
@@ -280,6 +298,10 @@ struct jfc_graph {
     dfree(d_hmat);
     dfree(d_touched);
     dfree(d_lines);
+    dfree(rt.d_inst); dfree(rt.d_tile_inst); dfree(rt.d_dirty); dfree(rt.d_defi);
+    dfree(rt.d_jf0); dfree(rt.d_jf1); dfree(rt.d_tdist); dfree(rt.d_hist); dfree(rt.d_order);
+    dfree(rt.d_lines);
+    rt = RelabelTiles{};
   }
   void release_aux() {
     if (aux_fork) (void)hipEventDestroy(aux_fork);
@@ -418,8 +440,8 @@ struct jfc_graph {
     if ((rc = dalloc(&d_inst, inst.size(), "instances")) ||
         (rc = dalloc(&d_tile_inst, ntiles, "tile table")) ||
         (rc = dalloc(&d_color_tiles, ntiles, "colour classes")) ||
-        (rc = dalloc(&d_list[0], ntiles, "tile list")) ||
-        (rc = dalloc(&d_list[1], ntiles, "tile list")) ||
+        (rc = dalloc(&d_list[0], list_cap(sh), "tile list")) ||
+        (rc = dalloc(&d_list[1], list_cap(sh), "tile list")) ||
         (rc = dalloc(&d_flag, ntiles, "tile flags")) ||
         (rc = dalloc(&d_dirty, ntiles, "tile flags")) ||
         (rc = dalloc(&d_seed, ntiles, "tile seeds")) ||
@@ -438,7 +460,112 @@ struct jfc_graph {
     JFC_HIP(hipMemcpy(d_inst, inst.data(), inst.size() * sizeof(InstDesc), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_tile_inst, tile_inst.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
     JFC_HIP(hipMemcpy(d_color_tiles, flat.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+    return sh == SH_32x4x2 ? build_relabel_tiles(SH_8x8x4) : JFC_OK;
+  }
+
+  // Tile count of shape sh over the instances.
+  long long count_tiles(int sh) const {
+    int tw, th, td;
+    shape_dims(sh, &tw, &th, &td);
+    long long n = 0;
+    for (const auto &I : inst)
+      n += (long long)((I.nx + tw - 1) / tw) * ((I.ny + th - 1) / th) * ((I.nz + td - 1) / td);
+    return n;
+  }
+  // Capacity of the compaction lists: the tiles of the push tiling, or of
+  // the relabel tiling where that is another.
+  long long list_cap(int sh) const {
+    return sh == SH_32x4x2 ? std::max(count_tiles(sh), count_tiles(SH_8x8x4)) : count_tiles(sh);
+  }
+
+  // The relabel tiling's tables (rt) for shape sh2 over the same instances:
+  // descriptors, tile -> instance table, tile lines and the search buffers.
+  int build_relabel_tiles(int sh2) {
+    rt.shape = sh2;
+    int tw, th, td;
+    shape_dims(sh2, &tw, &th, &td);
+    rt.inst = inst;
+    long long off = 0;
+    int maxsum = 0;
+    rt.max_tile_dim = 1;
+    for (auto &I : rt.inst) {
+      I.tx = (I.nx + tw - 1) / tw;
+      I.ty = (I.ny + th - 1) / th;
+      I.tz = (I.nz + td - 1) / td;
+      const long long nt = (long long)I.tx * I.ty * I.tz;
+      if (off + nt >= kEntryTile) return fail(JFC_ERR_INVALID, "graph too large for the tile queue");
+      I.tile_off = (int)off;
+      I.ntiles = (int)nt;
+      off += nt;
+      rt.max_tile_dim = std::max(rt.max_tile_dim, std::max(I.tx, std::max(I.ty, I.tz)));
+      maxsum = std::max(maxsum, I.tx + I.ty + I.tz);
+    }
+    rt.ntiles = (int)off;
+    rt.nbins = std::min(maxsum + 2, 4096);
+    std::vector<int> tile_inst(rt.ntiles);
+    const int n1 = (int)rt.inst.size() + 1;
+    std::vector<int> lines(3 * n1, 0);
+    for (size_t i = 0; i < rt.inst.size(); ++i) {
+      const InstDesc &I = rt.inst[i];
+      std::fill(tile_inst.begin() + I.tile_off, tile_inst.begin() + I.tile_off + I.ntiles, (int)i);
+      const long long per[3] = {(long long)I.ty * I.tz, (long long)I.tx * I.tz,
+                                (long long)I.tx * I.ty};
+      for (int a = 0; a < 3; ++a) {
+        const long long nx = lines[a * n1 + i] + per[a];
+        if (nx >= (1LL << 31) / 4) return fail(JFC_ERR_INVALID, "too many tile lines");
+        lines[a * n1 + i + 1] = (int)nx;
+      }
+    }
+    for (int a = 0; a < 3; ++a) rt.lines_total[a] = lines[a * n1 + n1 - 1];
+    int rc;
+    if ((rc = dalloc(&rt.d_inst, rt.inst.size(), "instances")) ||
+        (rc = dalloc(&rt.d_tile_inst, rt.ntiles, "tile table")) ||
+        (rc = dalloc(&rt.d_dirty, rt.ntiles, "tile flags")) ||
+        (rc = dalloc(&rt.d_defi, rt.ntiles, "tile seeds")) ||
+        (rc = dalloc(&rt.d_jf0, rt.ntiles, "jump flooding")) ||
+        (rc = dalloc(&rt.d_jf1, rt.ntiles, "jump flooding")) ||
+        (rc = dalloc(&rt.d_tdist, rt.ntiles, "tile distance")) ||
+        (rc = dalloc(&rt.d_hist, rt.nbins, "tile histogram")) ||
+        (rc = dalloc(&rt.d_order, rt.ntiles, "tile order")) ||
+        (rc = dalloc(&rt.d_lines, 3 * n1, "tile lines")))
+      return rc;
+    JFC_HIP(hipMemcpy(rt.d_lines, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpy(rt.d_inst, rt.inst.data(), rt.inst.size() * sizeof(InstDesc),
+                      hipMemcpyHostToDevice));
+    JFC_HIP(hipMemcpy(rt.d_tile_inst, tile_inst.data(), rt.ntiles * sizeof(int),
+                      hipMemcpyHostToDevice));
     return JFC_OK;
+  }
+
+  // The tiling a search runs on (the relabel tiling for relabels, else the
+  // push tiling).
+  struct SearchTiling {
+    int shape;
+    Tiles T;
+    int ntiles, nbins, max_tile_dim;
+    unsigned char *dirty, *defi;
+    int *jf0, *jf1, *tdist, *hist, *order;
+    TileLines lines;
+    const InstDesc *pinst;  // push tiling descriptors when the tiling is another
+    int pdims;              // push tile dimensions, log2 (x | y << 4 | z << 8)
+  };
+  SearchTiling search_tiling(bool relabel) const {
+    if (!relabel || !rt.shape)
+      return SearchTiling{shape, tiles(), ntiles, nbins, max_tile_dim, d_dirty, d_defi, d_jf0,
+                          d_jf1, d_tdist, d_hist, d_order, lines(), nullptr, 0};
+    Tiles T = tiles();
+    T.inst = rt.d_inst;
+    T.tile_inst = rt.d_tile_inst;
+    T.ntiles = rt.ntiles;
+    T.one = rt.inst.empty() ? InstDesc{} : rt.inst[0];
+    const int n1 = (int)rt.inst.size() + 1;
+    const TileLines L{{rt.d_lines, rt.d_lines + n1, rt.d_lines + 2 * n1},
+                      {rt.lines_total[0], rt.lines_total[1], rt.lines_total[2]}};
+    int tw, th, td;
+    shape_dims(shape, &tw, &th, &td);
+    const int pdims = __builtin_ctz(tw) | __builtin_ctz(th) << 4 | __builtin_ctz(td) << 8;
+    return SearchTiling{rt.shape, T, rt.ntiles, rt.nbins, rt.max_tile_dim, rt.d_dirty, rt.d_defi,
+                        rt.d_jf0, rt.d_jf1, rt.d_tdist, rt.d_hist, rt.d_order, L, d_inst, pdims};
   }
 };
 
@@ -1006,7 +1133,8 @@ struct Solver {
   // cleared: the caller has zeroed the dirty flags and the distance
   // histogram already (search_fills).
   int search(int mode, int *passes, bool cleared = false) {
-    const Tiles T = g->tiles();
+    const jfc_graph::SearchTiling ST = g->search_tiling(mode == kRelabel);
+    const Tiles &T = ST.T;
     Planes P = g->planes();
     if (mode == kTag) P.lab = g->tag;
     // Relabel passes take the tiles in distance order from the tiles holding
@@ -1019,34 +1147,35 @@ struct Solver {
     // of at most kSmallTiles tiles gives every tile a workgroup of its own in
     // a pass: those take the tiles in colour order and skip the five launches
     // of the distance ordering.
-    int *order = g->d_order;
-    if (g->ntiles <= kSmallTiles || mode != kRelabel) {
+    // (A relabel tiling other than the push tiling has no colour lists: its
+    // relabels always take the distance order.)
+    int *order = ST.order;
+    if ((ST.ntiles <= kSmallTiles && !ST.pinst) || mode != kRelabel) {
       order = g->d_color_tiles;
     } else {
-      launch_tile_order(g->shape, T, mode == kRelabel ? g->d_defi : g->d_seed, g->d_jf0,
-                        g->d_jf1, g->d_tdist, g->d_hist, g->nbins, g->d_order, g->max_tile_dim,
-                        nullptr, g->lines(), o.tile_order, st, cleared);
+      launch_tile_order(ST.shape, T, ST.defi, ST.jf0, ST.jf1, ST.tdist, ST.hist, ST.nbins,
+                        ST.order, ST.max_tile_dim, nullptr, ST.lines, o.tile_order, st, cleared);
     }
-    if (!cleared) launch_fill_u8(g->d_dirty, g->ntiles, 0, st);
-    const int nwg = std::min(g->nwg, g->ntiles);
+    if (!cleared) launch_fill_u8(ST.dirty, ST.ntiles, 0, st);
+    const int nwg = std::min(g->nwg, ST.ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
     if (int rc = g->ensure_pin()) return rc;
     auto flood = [&](const Queue &q, bool full_pass) {
-      if (mode == kCut) launch_reach(g->shape, nwg, T, P, q, g->d_dirty, g->d_flagword, st);
-      else if (mode == kTag) launch_tag(g->shape, nwg, T, P, q, g->d_dirty, g->lab, st);
-      else launch_relax(g->shape, nwg, T, P, q, g->d_dirty, g->d_flag, full_pass && g->dim != 3,
-                        g->gr_tag, st);
+      if (mode == kCut) launch_reach(ST.shape, nwg, T, P, q, ST.dirty, g->d_flagword, st);
+      else if (mode == kTag) launch_tag(ST.shape, nwg, T, P, q, ST.dirty, g->lab, st);
+      else launch_relax(ST.shape, nwg, T, P, q, ST.dirty, g->d_flag, full_pass && g->dim != 3,
+                        g->gr_tag, st, ST.pinst, ST.pdims);
     };
     if (mode != kRelabel) {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
       Queue q = queue_from_list();
-      launch_compact(order, g->ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
+      launch_compact(order, ST.ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
                      nullptr, st);
       flood(q, false);
     } else {
-      flood(Queue{order, g->d_ctr, g->slot, g->ntiles}, true);
+      flood(Queue{order, g->d_ctr, g->slot, ST.ntiles}, true);
     }
     advance();
     *passes += 1;
@@ -1063,7 +1192,7 @@ struct Solver {
       const int size = k == 0 ? 2 : kPassGroup;
       for (int j = 0; j < size; ++j, ++pass) {
         Queue qc = queue_from_list();
-        launch_compact(order, g->ntiles, pass & 1, g->d_dirty, nullptr, nullptr, qc, nullptr,
+        launch_compact(order, ST.ntiles, pass & 1, ST.dirty, nullptr, nullptr, qc, nullptr,
                        total, nullptr, st);
         flood(qc, false);
         advance();
@@ -1105,22 +1234,26 @@ struct Solver {
   // deficit flags) config 3 took 14.7 ms instead of 13.4.
   // The zeroing a search needs (dirty flags, distance histogram), to go into
   // its caller's fill launch.
-  void search_fills(FillSet &f) const {
-    f.add(g->d_hist, g->nbins, 0, 4);
-    f.add(g->d_dirty, g->ntiles, 0, 1);
+  void search_fills(FillSet &f, bool relabel = false) const {
+    const jfc_graph::SearchTiling ST = g->search_tiling(relabel);
+    f.add(ST.hist, ST.nbins, 0, 4);
+    f.add(ST.dirty, ST.ntiles, 0, 1);
   }
 
   int global_relabel() {
     hipEvent_t a = tm.mark();
     g->gr_tag = 3 - g->gr_tag;
+    const jfc_graph::SearchTiling ST = g->search_tiling(true);
     {
       FillSet f;
-      search_fills(f);
-      if (g->dim == 3) f.add(g->d_defi, g->ntiles, 0, 1);
+      search_fills(f, true);
+      if (g->dim == 3) f.add(ST.defi, ST.ntiles, 0, 1);
+      // the relax passes set the push tiling's active flags, never clear them
+      if (ST.pinst) f.add(g->d_flag, g->ntiles, 0, 1);
       launch_fill_multi(f, st);
     }
     if (g->dim == 3)
-      launch_gr_init(g->shape, g->tiles(), g->planes(), g->gr_tag, g->d_defi, st, true);
+      launch_gr_init(ST.shape, ST.T, g->planes(), g->gr_tag, ST.defi, st, true);
     int rc = search(kRelabel, &s.relabel_passes, true);
     if (rc) return rc;
     launch_fill_u8(g->d_hmat, g->ntiles, 1, st);  // every height is stored now

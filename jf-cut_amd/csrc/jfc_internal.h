@@ -247,10 +247,16 @@ void launch_gr_init(int shape, const Tiles &T, const Planes &P, int tag, unsigne
 // One relabel pass.  The first pass (first_pass, every tile) starts each
 // tile from e (0 at deficits, HINF elsewhere) and writes every height word
 // with `tag`; later passes, and every pass after launch_gr_init, relax the
-// queued tiles from their stored heights.
+// queued tiles from their stored heights.  `active` holds the tiles' active
+// flags; with `pinst` (3D shapes only: a relabel tiling other than the push tiling) it
+// is indexed by the push tiling instead -- pinst its instance descriptors,
+// pdims its tile dimensions as log2 (x | y << 4 | z << 8) -- and every
+// visit sets the flags of the push tiles holding active nodes (the caller
+// clears them before the relabel).
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  bool first_pass, int tag, hipStream_t st);
+                  bool first_pass, int tag, hipStream_t st, const InstDesc *pinst = nullptr,
+                  int pdims = 0);
 void launch_reach(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, int *cert,
                   hipStream_t st);

@@ -1764,7 +1764,8 @@ void launch_push(int shape, int nwg, const Tiles &T, const Planes &P,
 template <class S, bool FIRST, bool ONE>
 __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
                                                unsigned char *dirty,
-                                               unsigned char *active, int tag) {
+                                               unsigned char *active, int tag,
+                                               const InstDesc *pinst, int pdims) {
   constexpr bool first = FIRST;
   __shared__ int s_h[S::PADN];
   const int tid = threadIdx.x;
@@ -1870,8 +1871,20 @@ __global__ __launch_bounds__(256) void k_relax(Tiles T, Planes P, Queue q,
     // graphcut_count_node fused in: the tile's last visit leaves its
     // active flag (a node with excess and a finite height)
     if (tid == 0) Qc.publish();
-    const int act = block_any(valid && e0 > 0 && s_h[L.me] < HINF, s_w, par);
-    if (tid == 0) active[t] = (unsigned char)act;
+    const bool on = valid && e0 > 0 && s_h[L.me] < HINF;
+    const int act = block_any(on, s_w, par);
+    if (pinst) {
+      // flags of the push tiling: each active node sets its push tile's.
+      // During a relabel e is fixed and heights only fall, so a node once
+      // active stays active, and setting flags (never clearing) is exact.
+      if (on) {
+        const InstDesc &PI = pinst[ONE ? 0 : T.tile_inst[t]];
+        active[PI.tile_off + (gx >> (pdims & 15)) +
+               PI.tx * ((gy >> ((pdims >> 4) & 15)) + PI.ty * (gz >> (pdims >> 8)))] = 1;
+      }
+    } else if (tid == 0) {
+      active[t] = (unsigned char)act;
+    }
     Qc.advance();
   }
 }
@@ -2026,10 +2039,12 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
 // relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
 template <bool FIRST, bool ONE>
 static void relax_launch(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
-                         unsigned char *dirty, unsigned char *active, int tag, hipStream_t st) {
+                         unsigned char *dirty, unsigned char *active, int tag, hipStream_t st,
+                         const InstDesc *pinst, int pdims) {
   if (shape_is_3d(shape)) {
     JFC_DISPATCH(shape, (k_relax<S, FIRST, ONE><<<resident_grid<&k_relax<S, FIRST, ONE>>(nwg),
-                                                  256, 0, st>>>(T, P, q, dirty, active, tag)));
+                                                  256, 0, st>>>(T, P, q, dirty, active, tag,
+                                                                pinst, pdims)));
   } else {
     JFC_DISPATCH(shape,
                  (k_relax_w<S, FIRST, ONE><<<resident_grid_b<&k_relax_w<S, FIRST, ONE>, 64>(
@@ -2040,13 +2055,15 @@ static void relax_launch(int shape, int nwg, const Tiles &T, const Planes &P, co
 
 void launch_relax(int shape, int nwg, const Tiles &T, const Planes &P,
                   const Queue &q, unsigned char *dirty, unsigned char *active,
-                  bool first_pass, int tag, hipStream_t st) {
+                  bool first_pass, int tag, hipStream_t st, const InstDesc *pinst, int pdims) {
   if (first_pass) {
-    if (T.ninst == 1) relax_launch<true, true>(shape, nwg, T, P, q, dirty, active, tag, st);
-    else relax_launch<true, false>(shape, nwg, T, P, q, dirty, active, tag, st);
+    if (T.ninst == 1)
+      relax_launch<true, true>(shape, nwg, T, P, q, dirty, active, tag, st, pinst, pdims);
+    else relax_launch<true, false>(shape, nwg, T, P, q, dirty, active, tag, st, pinst, pdims);
   } else {
-    if (T.ninst == 1) relax_launch<false, true>(shape, nwg, T, P, q, dirty, active, tag, st);
-    else relax_launch<false, false>(shape, nwg, T, P, q, dirty, active, tag, st);
+    if (T.ninst == 1)
+      relax_launch<false, true>(shape, nwg, T, P, q, dirty, active, tag, st, pinst, pdims);
+    else relax_launch<false, false>(shape, nwg, T, P, q, dirty, active, tag, st, pinst, pdims);
   }
 }
 
