@@ -223,6 +223,39 @@ def test_3d_tile_shape_option(gpu):
     assert g.flow() == f and np.array_equal(g.labels(), l)
 
 
+@pytest.mark.parametrize("opts", [dict(initial_relabel=1), dict(initial_relabel=1, relabel_work=0.01),
+                                  dict(initial_relabel=0, relabel_work=0.01, max_local_iters=1)])
+def test_32x4x2_relabels_on_8x8x4_tiles_gpu(gpu, opts):
+    """Graphs on 32x4x2 push tiles relabel on an 8x8x4 tiling, and the relax
+    visits set the push tiling's active flags.  Sizes cut both tilings
+    unevenly (x not a multiple of 32, y of 8, z of 4); one graph and a batch
+    of instances (the per-instance descriptors of both tilings), each
+    relabelled at least once, against the oracle's flow and labels."""
+    rng = np.random.default_rng(77)
+    shp = [(70, 21, 9, 6), (33, 9, 5, 6), (40, 17, 3, 6), (96, 12, 7, 6)]
+    data = [random_grid(rng, a, b, c, 6, seed_frac=0.02) for (a, b, c, _) in shp]
+    exp = [ol.maxflow(a, b, c, 6, *data[i])[:2] for i, (a, b, c, _) in enumerate(shp)]
+    say("case 32x4x2 single", shp[0], opts)
+    relabelled = opts["initial_relabel"] == 1  # otherwise the visit budget triggers them
+    g, st = solve_grid(gpu, *shp[0][:3], 6, *data[0], tile_shape="32x4x2", **opts)
+    assert st.converged == 1 and (st.global_relabels >= 1 or not relabelled)
+    assert g.flow() == exp[0][0] and np.array_equal(g.labels(), exp[0][1])
+    g.close()
+    say("case 32x4x2 batch", shp, opts)
+    G = gpu.GridGraph(shp)
+    for i, (cs, ct, caps) in enumerate(data):
+        G.set_terminal_caps(cs, ct, inst=i)
+        for d in range(6):
+            G.set_neighbor_caps(d, caps[d], inst=i)
+    for rep in range(2):  # the second solve starts from the first's state
+        st = G.solve(tile_shape="32x4x2", **opts)
+        assert st.converged == 1 and (st.global_relabels >= 1 or not relabelled)
+        for i, (ff, ll) in enumerate(exp):
+            assert G.flow(i) == ff, (rep, i)
+            assert np.array_equal(G.labels(i), ll), (rep, i)
+    G.close()
+
+
 def test_c1_goldens_gpu(gpu):
     """Config 1 end to end on the device: data term (k-means, unary, pairwise)
     bit-exact with the oracle, then flow and labels equal to the goldens."""
