@@ -1430,6 +1430,17 @@ constexpr int kRingQ = (Ring<S>::N + 3) / 4;
 template <class S>
 constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 
+// A 3D ring cell (a face cell of the padded tile) has one arc into the tile
+// (6-connected arcs cross faces only): its direction from the cell, which is
+// also the one plane of the cell that flow out of the tile changes.
+template <class S>
+__device__ __forceinline__ int inward_dir(int p) {
+  int x, y, z, nout;
+  bool inter;
+  pad_coords<S>(p, x, y, z, inter, nout);
+  return x < 0 ? 0 : x >= S::TW ? 1 : y < 0 ? 2 : y >= S::TH ? 3 : z < 0 ? 4 : 5;
+}
+
 // ---- block-wise push-relabel -------------------------------------------------------
 //
 // One workgroup discharges one tile at a time: e, h and the K residual planes
@@ -1452,14 +1463,20 @@ constexpr int kRingPerLane = (kRingQ<S> + 63) / 64;
 template <class S, bool ONE>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
 // without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
-// 8, where it spills).  Two ring cells per lane (32x4x2) take ~125: 4 waves.
-__global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
+// 8, where it spills).  Two ring cells per lane (32x4x2, slim flush) fit 96: 5 waves.
+__global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 5 : 7) void k_push(
     Tiles T, Planes P, Queue q, unsigned char *flag, unsigned char *mat, unsigned char *touched,
     unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
     const int *sweep_state, long long *iter_total) {
   __shared__ int s_h[S::PADN];
   __shared__ int s_e[S::PADN];
   __shared__ int s_c[S::KMAX][S::PADN];
+  // Two ring cells per lane (3D 32x4x2): the flush finds its cells again
+  // from the lane index and their mask words in LDS, and takes flow back on
+  // a cell's one inward plane, so no ring state is held in registers
+  // through the local iterations (4 waves per SIMD otherwise).
+  constexpr bool kSlim = S::D3 && kRingPerLane<S> > 1;
+  __shared__ unsigned s_hm[kSlim ? S::PADN : 1];
   const int tid = threadIdx.x;
   const Lane<S> L(tid);
   __shared__ int s_next[3];
@@ -1568,6 +1585,7 @@ __global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
       const int p = hp[c];
       s_h[p] = hc[c].live ? hh[c] : HINF;
       s_e[p] = 0;
+      if constexpr (kSlim) s_hm[p] = hm[c];
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) s_c[d][p] = 0;
     }
@@ -1648,17 +1666,45 @@ __global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
     // to, loaded before the interior write-back so that its stores cover the
     // load latency.
     int fl[kRingPerLane<S>];
-    int dc[kRingPerLane<S>][S::KMAX], cv[kRingPerLane<S>][S::KMAX];
+    int dc[kRingPerLane<S>][kSlim ? 1 : S::KMAX], cv[kRingPerLane<S>][kSlim ? 1 : S::KMAX];
     unsigned fr[kRingPerLane<S>], fp[kRingPerLane<S>];
+    if constexpr (kSlim) {
+      // the ring cells found again (an opaque copy of the lane index keeps
+      // the compiler from holding the load phase's values instead)
+      int tid2 = tid;
+      asm volatile("" : "+v"(tid2));
+#pragma unroll
+      for (int c = 0; c < kRingPerLane<S>; ++c) {
+        const int j = c * 64 + (tid2 & 63);
+        const int k = (tid2 >> 6) * kRingQ<S> + j;
+        hp[c] = j < kRingQ<S> && k < Ring<S>::N ? Ring<S>::pos(k) : S::PADN;
+        hc[c] = halo_cell<S>(I, r.ox, r.oy, r.oz, hp[c], K);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < kRingPerLane<S>; ++c) {
       fl[c] = 0;
       fr[c] = fp[c] = 0;
+      if constexpr (!kSlim) {
 #pragma unroll
-      for (int d = 0; d < S::KMAX; ++d) dc[c][d] = cv[c][d] = 0;
+        for (int d = 0; d < S::KMAX; ++d) dc[c][d] = cv[c][d] = 0;
+      } else {
+        dc[c][0] = cv[c][0] = 0;
+      }
       if (!hc[c].live) continue;
       const int p = hp[c];
       fl[c] = s_e[p];
+      if constexpr (kSlim) {
+        const int d = inward_dir<S>(p);
+        const unsigned m = s_hm[p];
+        dc[c][0] = s_c[d][p];
+        if (dc[c][0] != 0) {
+          if ((m >> (8 + d)) & 1u) fr[c] = 1u << d;
+          if (!((m >> d) & 1u)) fp[c] = 1u << d;
+        }
+        if (fr[c]) cv[c][0] = P.c0[d * st + hc[c].g];
+        continue;
+      } else {
 #pragma unroll
       for (int d = 0; d < S::KMAX; ++d) {
         dc[c][d] = d < K ? s_c[d][p] : 0;
@@ -1671,6 +1717,7 @@ __global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
         const long long g = hc[c].g;
 #pragma unroll
         for (int d = 0; d < S::KMAX; ++d) cv[c][d] = P.c0[((fr[c] >> d) & 1u) ? d * st + g : 0];
+      }
       }
     }
 
@@ -1716,10 +1763,16 @@ __global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 4 : 7) void k_push(
         flag[ht] = 1;
         touched[ht] = 1;
       }
+      if constexpr (kSlim) {
+        const long long a = inward_dir<S>(p) * st + g;
+        if (fr[c]) P.cap[a] = cv[c][0] + dc[c][0];
+        else if (dc[c][0] != 0) atomicAdd(&P.cap[a], dc[c][0]);
+      } else {
 #pragma unroll
-      for (int d = 0; d < S::KMAX; ++d) {
-        if ((fr[c] >> d) & 1u) P.cap[d * st + g] = cv[c][d] + dc[c][d];
-        else if (dc[c][d] != 0) atomicAdd(&P.cap[d * st + g], dc[c][d]);
+        for (int d = 0; d < S::KMAX; ++d) {
+          if ((fr[c] >> d) & 1u) P.cap[d * st + g] = cv[c][d] + dc[c][d];
+          else if (dc[c][d] != 0) atomicAdd(&P.cap[d * st + g], dc[c][d]);
+        }
       }
       const unsigned flip = fp[c] | (fr[c] << 8);
       if (flip) atomicXor(&P.rmask[g], flip);
