@@ -163,6 +163,7 @@ struct jfc_graph {
   int *d_list[2] = {nullptr, nullptr};
   int *d_ctr = nullptr;
   unsigned char *d_flag = nullptr, *d_dirty = nullptr, *d_seed = nullptr;
+  unsigned char *d_dirty2 = nullptr;  // second dirty buffer of small-graph searches
   unsigned char *d_defi = nullptr;  // tiles that may hold a deficit node
   int gr_tag = 1;                   // tag of the last global relabel's heights
   unsigned char *d_mat = nullptr;  // per tile: residual planes written this solve
@@ -293,7 +294,8 @@ struct jfc_graph {
   void release_tiles() {
     drop_graphs();
     dfree(d_tile_inst); dfree(d_color_tiles); dfree(d_list[0]); dfree(d_list[1]);
-    dfree(d_flag); dfree(d_dirty); dfree(d_seed); dfree(d_defi); dfree(d_jf0); dfree(d_jf1);
+    dfree(d_flag); dfree(d_dirty); dfree(d_dirty2); dfree(d_seed); dfree(d_defi); dfree(d_jf0);
+    dfree(d_jf1);
     dfree(d_tdist); dfree(d_hist); dfree(d_order); dfree(d_inst); dfree(d_mat);
     dfree(d_hmat);
     dfree(d_touched);
@@ -444,6 +446,7 @@ struct jfc_graph {
         (rc = dalloc(&d_list[1], list_cap(sh), "tile list")) ||
         (rc = dalloc(&d_flag, ntiles, "tile flags")) ||
         (rc = dalloc(&d_dirty, ntiles, "tile flags")) ||
+        (rc = dalloc(&d_dirty2, ntiles, "tile flags")) ||
         (rc = dalloc(&d_seed, ntiles, "tile seeds")) ||
         (rc = dalloc(&d_defi, ntiles, "tile seeds")) ||
         (rc = dalloc(&d_jf0, ntiles, "jump flooding")) ||
@@ -1156,26 +1159,46 @@ struct Solver {
       launch_tile_order(ST.shape, T, ST.defi, ST.jf0, ST.jf1, ST.tdist, ST.hist, ST.nbins,
                         ST.order, ST.max_tile_dim, nullptr, ST.lines, o.tile_order, st, cleared);
     }
-    if (!cleared) launch_fill_u8(ST.dirty, ST.ntiles, 0, st);
+    // A small graph's passes after the first take their tiles straight from
+    // the flags (Queue::pick over the colour lists, no compaction launch).
+    // The flags are then double-buffered: pass p takes the tiles flagged in
+    // D[p & 1] and flags the next pass's in D[(p + 1) & 1], so a flag set
+    // during a pass is read only after the launch boundary that makes the
+    // labels or heights behind it visible.
+    const bool direct = ST.ntiles <= kSmallTiles && !ST.pinst;
+    unsigned char *const D[2] = {direct ? g->d_dirty2 : ST.dirty, ST.dirty};
+    if (!cleared) {
+      launch_fill_u8(ST.dirty, ST.ntiles, 0, st);
+      if (direct) launch_fill_u8(g->d_dirty2, ST.ntiles, 0, st);
+    }
     const int nwg = std::min(g->nwg, ST.ntiles);
     // totals: [1] relabel tiles, [2] cut tiles, [4] tag tiles
     long long *total = g->d_total + (mode == kTag ? 4 : 1 + mode);
     if (int rc = g->ensure_pin()) return rc;
-    auto flood = [&](const Queue &q, bool full_pass) {
-      if (mode == kCut) launch_reach(ST.shape, nwg, T, P, q, ST.dirty, g->d_flagword, st);
-      else if (mode == kTag) launch_tag(ST.shape, nwg, T, P, q, ST.dirty, g->lab, st);
-      else launch_relax(ST.shape, nwg, T, P, q, ST.dirty, g->d_flag, full_pass && g->dim != 3,
+    // dirty: the flags this pass sets for the next
+    auto flood = [&](const Queue &q, bool full_pass, unsigned char *dirty) {
+      if (mode == kCut) launch_reach(ST.shape, nwg, T, P, q, dirty, g->d_flagword, st);
+      else if (mode == kTag) launch_tag(ST.shape, nwg, T, P, q, dirty, g->lab, st);
+      else launch_relax(ST.shape, nwg, T, P, q, dirty, g->d_flag, full_pass && g->dim != 3,
                         g->gr_tag, st, ST.pinst, ST.pdims);
     };
     if (mode != kRelabel) {
       // tiles without a seed gain labels only through their halo, so the
       // first pass visits the seed tiles (in distance order)
-      Queue q = queue_from_list();
-      launch_compact(order, ST.ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
-                     nullptr, st);
-      flood(q, false);
+      if (direct) {
+        Queue qd{order, g->d_ctr, g->slot, ST.ntiles};
+        qd.pick = g->d_seed;
+        qd.cnt = g->d_ctr + QSLOT * g->slot;
+        qd.total = total;
+        flood(qd, false, D[1]);
+      } else {
+        Queue q = queue_from_list();
+        launch_compact(order, ST.ntiles, 0, g->d_seed, nullptr, nullptr, q, nullptr, total,
+                       nullptr, st);
+        flood(q, false, D[1]);
+      }
     } else {
-      flood(Queue{order, g->d_ctr, g->slot, ST.ntiles}, true);
+      flood(Queue{order, g->d_ctr, g->slot, ST.ntiles}, true, D[1]);
     }
     advance();
     *passes += 1;
@@ -1191,14 +1214,24 @@ struct Solver {
       // pass or two, and passes queued past the end each cost two launches
       const int size = k == 0 ? 2 : kPassGroup;
       for (int j = 0; j < size; ++j, ++pass) {
-        Queue qc = queue_from_list();
-        launch_compact(order, ST.ntiles, pass & 1, ST.dirty, nullptr, nullptr, qc, nullptr,
-                       total, nullptr, st);
-        flood(qc, false);
+        if (direct) {
+          // the pass's visit count goes to its slot's count word (zeroed by
+          // the launch before, as a compaction's would be)
+          Queue qd{order, g->d_ctr, g->slot, ST.ntiles};
+          qd.pick = D[pass & 1];
+          qd.cnt = g->d_ctr + QSLOT * g->slot;
+          qd.total = total;
+          flood(qd, false, D[(pass + 1) & 1]);
+        } else {
+          Queue qc = queue_from_list();
+          launch_compact(order, ST.ntiles, pass & 1, ST.dirty, nullptr, nullptr, qc, nullptr,
+                         total, nullptr, st);
+          flood(qc, false, ST.dirty);
+        }
         advance();
         *passes += 1;
       }
-      // the last compaction's slot is the one before the current
+      // the last pass's slot is the one before the current
       JFC_HIP(hipMemcpyAsync(&g->pin->cnt[k & 1], g->d_ctr + QSLOT * (g->slot ^ 1), sizeof(int),
                              hipMemcpyDeviceToHost, st));
       JFC_HIP(hipEventRecord(g->cnt_ev[k & 1], st));
@@ -1238,6 +1271,7 @@ struct Solver {
     const jfc_graph::SearchTiling ST = g->search_tiling(relabel);
     f.add(ST.hist, ST.nbins, 0, 4);
     f.add(ST.dirty, ST.ntiles, 0, 1);
+    if (ST.ntiles <= kSmallTiles && !ST.pinst) f.add(g->d_dirty2, ST.ntiles, 0, 1);
   }
 
   int global_relabel() {
