@@ -96,7 +96,7 @@ typedef struct {
                              (default); 0 = launch them one by one.  profile = 1
                              always launches them one by one. */
   int tile_order;         /* tile distance field that orders the relabel
-                             passes of graphs of more than 1024 tiles: 0 =
+                             passes of graphs of more than 2048 tiles: 0 =
                              separable taxicab distance transform (default, D
                              launches), 1 = jump flooding
                              (graphcut_bfs_block.cl:43-63, 2 ceil(log2) + 2

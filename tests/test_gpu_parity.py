@@ -64,11 +64,12 @@ def test_random_grids_gpu(gpu, conn, shape):
         g.close()
 
 
-# 32x8 tiles: 1024 (small-graph paths: push launches pick their own tiles,
-# search passes in colour order), 1056 (colour classes still small enough for
-# self-picking pushes, relabel passes in distance order) and 8250 (compacted
-# push lists too)
-@pytest.mark.parametrize("conn,nx,ny", [(4, 1024, 256), (4, 1056, 256), (8, 1056, 256),
+# 32x8 tiles: 1024 (small-graph paths: push launches and search passes pick
+# their own tiles, search passes in colour order), 2080 (colour classes still
+# small enough for self-picking pushes; relabel passes in distance order,
+# compacted search lists) and 8250 (colour classes of 2062-2063 tiles:
+# compacted push lists too)
+@pytest.mark.parametrize("conn,nx,ny", [(4, 1024, 256), (4, 2080, 256), (8, 2080, 256),
                                         (8, 2100, 1000)])
 @pytest.mark.parametrize("opts", [dict(), dict(initial_relabel=1),
                                   dict(initial_relabel=1, stall_certify=1, tile_order=1)])
