@@ -159,7 +159,7 @@ def device_setter(g, n, K):
 
 
 def batched_forward_backward(n=256, reps=3):
-    """Config 5: the 256 uneven cuts (4C/8C, lambda 20-80) solved together and
+    """The 256 uneven cuts of preset c5 (4C/8C, lambda 20-80) solved together and
     differentiated in one backward launch, against the same cuts solved and
     differentiated one graph at a time.  Flows must agree exactly."""
     specs = jfcut.batch_specs(n, 0)
@@ -363,7 +363,7 @@ def config_leg(name, steps=3):
 
 
 def seeds_only_leg(W, H, steps=2):
-    """A harder 2D cut: the config-2 image and pairwise terms with hard seeds
+    """A harder 2D cut: the 15000x10000 workload's image and pairwise terms with hard seeds
     only (unary scale 0), so flow has to travel from the foreground scribbles
     across the image to the background scribbles."""
     wl = dict(WORKLOADS["c2"], W=W, H=H, unary=0)
@@ -385,7 +385,7 @@ def seeds_only_leg(W, H, steps=2):
 
 def fastcut_comparator(reps=5):
     """The paper's in-tree GPU comparison on this GPU: the same 4-connected
-    512x512 cut (config 1) solved by the tile push-relabel (JF-Cut) and by
+    512x512 cut (preset c1) solved by the tile push-relabel (JF-Cut) and by
     the Fast-Cut line pushes (jfc_options.method = 1); device ms per solve,
     flows equal."""
     wl = WORKLOADS["c1"]

@@ -99,8 +99,8 @@ struct Tuned {
 // Default cap on the local iterations of a launch whose list fits its grid,
 // from the solve's kSparseAfter-th sweep on (the reference fixtures: 4.6 ->
 // 2.2-3.0 ms against a cap of 2; from the first sweep on, larger caps slowed
-// graphs whose few active tiles iterate without effect, config 1 0.48 ->
-// 0.71 ms at 64).
+// graphs whose few active tiles iterate without effect, the 512x512
+// 4-connected cut 0.48 -> 0.71 ms at 64).
 constexpr int kSparseIters = 32;
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tune;
@@ -178,8 +178,8 @@ struct jfc_graph {
   // 32x4x2 push tiles read whole 128-byte rows (an 8x8x4 tile's rows are 32
   // bytes), but a relabel front crosses about one tile per pass, and 32x4x2
   // tiles are half as deep in y and z, so their relabels run on 8x8x4 tiles
-  // (config 3: push 1.48 ms on 32x4x2 tiles against 2.81 on 8x8x4, relabel
-  // 2.95 ms on 8x8x4 against 4.56 on 32x4x2).  shape 0: the relabel runs on
+  // (the 2560x1440x50 video volume: push 1.48 ms on 32x4x2 tiles against
+  // 2.81 on 8x8x4, relabel 2.95 ms on 8x8x4 against 4.56 on 32x4x2).  shape 0: the relabel runs on
   // the push tiling.  The relabel writes the push tiling's active flags.
   struct RelabelTiles {
     int shape = 0;
@@ -1148,8 +1148,9 @@ struct Solver {
     // visit reads its halo heights device-coherently, so a tile settled
     // earlier in the same pass is seen and one pass carries the front across
     // many tiles.  The cut and tag floods read their halo labels with plain
-    // loads and gain nothing from the order (config 2's cut: 0.58 ms ordered,
-    // 0.50 ms not, the same 9 passes; config 3: 0.74 -> 0.61 ms), and a graph
+    // loads and gain nothing from the order (the 15000x10000 8-connected
+    // cut's: 0.58 ms ordered, 0.50 ms not, the same 9 passes; the 2560x1440x50
+    // video volume's: 0.74 -> 0.61 ms), and a graph
     // of at most kSmallTiles tiles gives every tile a workgroup of its own in
     // a pass: those take the tiles in colour order and skip the five launches
     // of the distance ordering.
@@ -1263,11 +1264,12 @@ struct Solver {
   // graphcut_global_relabel.cl:19-124).  In 2D there is no separate init pass
   // over the nodes: the first tile pass starts every tile from e and marks
   // the heights it writes with a fresh tag, so a neighbour's stale height is
-  // recognised and replaced by its start value (config 2 relabel 1.03 ->
-  // 0.94 ms).  3D relabels keep the pass, which also makes the tile seeds
+  // recognised and replaced by its start value (a relabel of the
+  // 15000x10000 8-connected cut 1.03 -> 0.94 ms).  3D relabels keep the pass, which also makes the tile seeds
   // exact: 3D tiles have a halo as large as the tile, and without the pass
   // (halo excess read in the first tile pass, seeds from the push visits'
-  // deficit flags) config 3 took 14.7 ms instead of 13.4.
+  // deficit flags) the 2560x1440x50 video volume took 14.7 ms instead of
+  // 13.4.
   // The zeroing a search needs (dirty flags, distance histogram), to go into
   // its caller's fill launch.
   void search_fills(FillSet &f, bool relabel = false) const {
@@ -1814,13 +1816,15 @@ extern "C" int jfc_autotune(jfc_graph *g, const jfc_options *opt, int reps, int 
   const int nc = g->dim == 3 ? 2 : 3;
   // the local-iteration cap is tuned too unless the caller fixed it: fronts
   // that cross tiles in few visits favour more iterations per visit
-  // (config 4: 21.6 -> 14.3 ms at 4), others favour 2 (config 3)
+  // (the 512^3 volume: 21.6 -> 14.3 ms at 4), others favour 2 (the
+  // 2560x1440x50 video volume)
   const int mli_fixed = o.max_local_iters > 0 ? o.max_local_iters : 0;
   const int mcand[3] = {2, 3, 4};
   const int nm = mli_fixed ? 1 : 3;
   // so is the relabel before the first sweep: it pays where zero heights
-  // let flow slosh for sweeps (config 3: 12.2 -> 9.4 ms) and costs where
-  // the push sweeps settle most of the flow first (config 2: 4.8 -> 8.5 ms)
+  // let flow slosh for sweeps (the 2560x1440x50 video volume: 12.2 -> 9.4
+  // ms) and costs where the push sweeps settle most of the flow first (the
+  // 15000x10000 8-connected cut: 4.8 -> 8.5 ms)
   const int ir_fixed = o.initial_relabel >= 0 ? o.initial_relabel : -1;
   const int ni = ir_fixed >= 0 ? 1 : 2;
   // and whether a stalled active set first tries the certificate (it pays

@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_cut_init(Tiles T, Planes P, unsigned ch
   InstSums<1> sums;
   sums.reset();
   // two groups in flight per lane: 67 registers, 7 waves per SIMD (four: 95,
-  // 5 waves, 0.013 ms slower on a config-2 cut)
+  // 5 waves, 0.013 ms slower on the 15000x10000 8-connected cut)
   constexpr int U = 2;
   const int nsteps = row_steps(T);
   for (int step = blockIdx.x; step < nsteps; step += gridDim.x) {
@@ -1462,8 +1462,9 @@ __device__ __forceinline__ int inward_dir(int p) {
 // kernel arguments instead of loading it behind the tile index.
 template <class S, bool ONE>
 // 7 waves per SIMD: with one ring cell per lane the visit fits 69 VGPRs
-// without spills (config 2: 4.67 ms at 5 waves, 4.54 at 6, 4.48 at 7, 4.54 at
-// 8, where it spills).  Two ring cells per lane (32x4x2, slim flush) fit 96: 5 waves.
+// without spills (the 15000x10000 8-connected cut: 4.67 ms at 5 waves, 4.54
+// at 6, 4.48 at 7, 4.54 at 8, where it spills).  Two ring cells per lane
+// (32x4x2, slim flush) fit 96: 5 waves.
 __global__ __launch_bounds__(256, kRingPerLane<S> > 1 ? 5 : 7) void k_push(
     Tiles T, Planes P, Queue q, unsigned char *flag, unsigned char *mat, unsigned char *touched,
     unsigned char *defi, unsigned char *hmat, int max_iters, int sparse_iters,
@@ -2086,10 +2087,11 @@ __global__ __launch_bounds__(64) void k_relax_w(Tiles T, Planes P, Queue q,
   }
 }
 
-// 2D tiles: the wave-per-tile kernel (config 2's relabel 1.67 -> 1.21 ms).
+// 2D tiles: the wave-per-tile kernel (a relabel of the 15000x10000
+// 8-connected cut 1.67 -> 1.21 ms).
 // 3D tiles iterate more per visit (fronts cross 8x8x4 tiles in z as well)
-// and there the 256-lane visit's four parallel waves win (config 4's
-// relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
+// and there the 256-lane visit's four parallel waves win (the 512^3
+// volume's relabels 19 -> 15 ms), so 3D keeps the workgroup kernel.
 template <bool FIRST, bool ONE>
 static void relax_launch(int shape, int nwg, const Tiles &T, const Planes &P, const Queue &q,
                          unsigned char *dirty, unsigned char *active, int tag, hipStream_t st,

@@ -1,8 +1,9 @@
 """Full-size benchmark configurations on the GPU against goldens computed on
 the CPU by the oracle (and BK303 where the graph fits the build container):
 same image, same int32 capacity planes, same int64 flow, same canonical
-labels (sha256).  Plus the differentiable forward/backward of config 4 and
-the batched forward+backward of config 5."""
+labels (sha256).  Plus the differentiable forward/backward of the 512^3
+volume (preset c4) and the batched forward+backward of the 256 uneven cuts
+(preset c5)."""
 import hashlib
 import json
 import os
@@ -212,7 +213,7 @@ def test_config4_finite_differences_full_size(gpu):
 
 def test_config4_finite_differences(gpu):
     """+-1 integer finite differences on 10^4 non-tie arcs of crops of the
-    config-4 volume's own capacities: 22 crops of 32^3 nodes centred on arcs
+    512^3 volume's own capacities (preset c4): 22 crops of 32^3 nodes centred on arcs
     of the full solution's cut (arcs leaving a crop are dropped, so each crop
     is a graph of its own with its own min cut).  Each E(w +- 1) is an exact
     re-solve of the crop, 1024 perturbed crops solved per batched launch.

@@ -258,7 +258,7 @@ def test_32x4x2_relabels_on_8x8x4_tiles_gpu(gpu, opts):
 
 
 def test_c1_goldens_gpu(gpu):
-    """Config 1 end to end on the device: data term (k-means, unary, pairwise)
+    """The 512x512 4-connected cut (preset c1) end to end on the device: data term (k-means, unary, pairwise)
     bit-exact with the oracle, then flow and labels equal to the goldens."""
     for gd in load_c1_goldens():
         say("case c1 seed", gd["seed"])

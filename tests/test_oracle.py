@@ -254,7 +254,7 @@ def test_pair_thresholds_monotone_and_boundaries():
 
 
 def test_c1_goldens_cpu(jfcut_mod):
-    """Config-1 instances: the generator, the oracle's data term, the oracle's
+    """512x512 4-connected instances (preset c1): the generator, the oracle's data term, the oracle's
     solve and BK303 all agree with the committed goldens."""
     import hashlib
     sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
