@@ -210,7 +210,8 @@ struct jfc_graph {
   int *d_flagword = nullptr;  // in d_res; [0] certificate, [1] validation
   unsigned *mask0 = nullptr;  // residual-arc bits of the input planes
   unsigned *rmask = nullptr;  // residual-arc bits of the current residual
-  bool mask_dirty = true;     // inputs changed since mask0 was built
+  bool mask_dirty = true;     // inputs changed since the last solve started
+  bool mask0_ok = false;      // mask0 holds the bits of the current inputs
   int slot = 0;
   bool solved = false;
   std::vector<long long> flows;
@@ -1456,15 +1457,24 @@ struct Solver {
     // marked fresh: k_push reads fresh planes from c0 and writes them to cap
     const long long K = g->Kg, nodes = g->Nuser;
     // one pass over the nodes folds the terminal arcs (read cs, ct, write e)
-    // and starts the masks: rebuilt from the input planes after an input
-    // change (read K planes, write mask0 and rmask), else copied (read
-    // mask0, write rmask)
-    const int masks = g->mask_dirty ? 2 : 1;
+    // and starts the masks: built from the input planes after an input
+    // change (read K planes, write rmask), else copied from mask0 (read
+    // mask0, write rmask).  mask0 is written by the first solve that can
+    // reuse it -- the second solve of the same inputs -- so a solve of new
+    // inputs, the common case, skips that write (Fast-Cut reads mask0 at
+    // once and always has it written).
+    const bool fast = o.method == JFC_METHOD_FASTCUT;
+    int masks;
     if (g->mask_dirty) {
       g->mask_dirty = false;
       s.inputs_fresh = 1;
+      masks = fast ? 2 : 3;
+      g->mask0_ok = fast;
+    } else {
+      masks = g->mask0_ok ? 1 : 2;
+      g->mask0_ok = true;
     }
-    s.solve_bytes += nodes * (12 + (masks == 2 ? 4 * K + 8 : 8));
+    s.solve_bytes += nodes * (12 + (masks == 2 ? 4 * K + 8 : masks == 3 ? 4 * K + 4 : 8));
     if (int rc0 = g->ensure_pin()) return rc0;
     launch_fold(g->shape, T, P, g->d_red, o.initial_relabel ? nullptr : g->d_flag, g->d_defi,
                 o.method == JFC_METHOD_FASTCUT && !o.initial_relabel, g->mask0, masks, st, true);

@@ -468,15 +468,17 @@ static int row_grid(int nsteps) {
 //
 // The same pass starts the residual-arc masks of the solve (MASKS): 0 leaves
 // them alone, 1 copies mask0 to rmask, 2 builds mask0 from the input planes
-// (residual bit d = c0[d] > 0, every plane fresh) and writes it to both.  One
-// pass instead of a fold, a mask build and a copy: the mask words are
-// written once and the terminal planes read once.
+// (residual bit d = c0[d] > 0, every plane fresh) and writes it to both, 3
+// builds it into rmask only (mask0 is then built by the next solve of the
+// same inputs, if there is one).  One pass instead of a fold, a mask build
+// and a copy: the mask words are written once and the terminal planes read
+// once.
 template <class S, int MASKS>
 __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
                                               unsigned char *active, unsigned char *defi,
                                               int zero_h, unsigned *mask0) {
   // planes of eight directions in flight per group: one group per lane
-  constexpr int U = MASKS == 2 ? 1 : RS_U;
+  constexpr int U = MASKS >= 2 ? 1 : RS_U;
   __shared__ long long buf[4];
   long long *dst[3] = {red + RED_CONST, red + RED_DEF0, red + RED_SUP0};
   const int stride[3] = {RED_SLOTS, RED_SLOTS, RED_SLOTS};
@@ -486,7 +488,7 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
   for (int step = blockIdx.x; step < nsteps; step += gridDim.x) {
     const RowStep<S> RS(T, step);
     sums.enter(RS.R0.inst, dst, stride, buf);
-    const int K = MASKS == 2 ? T.inst[RS.R0.inst].K : 0;
+    const int K = MASKS >= 2 ? T.inst[RS.R0.inst].K : 0;
     for (int k0 = threadIdx.x; k0 < RS.slots; k0 += U * 256) {
       Group g[U];
       RowRef<S> R[U];
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(256) void k_fold(Tiles T, Planes P, long long *red,
         a[u] = load4(P.cs, R[u].v0 + g[u].x, g[u].n);
         b[u] = load4(P.ct, R[u].v0 + g[u].x, g[u].n);
         if (MASKS == 1) m[u] = load4((const int *)mask0, R[u].v0 + g[u].x, g[u].n);
-        if (MASKS == 2) {
+        if (MASKS >= 2) {
           int4 c[S::KMAX];
 #pragma unroll
           for (int d = 0; d < S::KMAX; ++d)
@@ -551,7 +553,10 @@ void launch_fold(int shape, const Tiles &T, const Planes &P, long long *red,
   }
   const int z = zero_h ? 1 : 0;
   const int n = (T.nrows + T.row_step - 1) / T.row_step;
-  if (masks == 2) {
+  if (masks == 3) {
+    JFC_DISPATCH(shape, (k_fold<S, 3><<<row_grid<&k_fold<S, 3>>(n), 256, 0, st>>>(
+                            T, P, red, active, defi, z, mask0)));
+  } else if (masks == 2) {
     JFC_DISPATCH(shape, (k_fold<S, 2><<<row_grid<&k_fold<S, 2>>(n), 256, 0, st>>>(
                             T, P, red, active, defi, z, mask0)));
   } else if (masks == 1) {
