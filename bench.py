@@ -12,9 +12,9 @@ rebuilds everything derived from them (residual-arc masks, folded excess) and
 runs to the exact int64 flow and the canonical labels in HBM, as a caller
 runs it (no timing events between launches).  The push kernel's launch
 durations come from a second timed run of the same fresh-input solve with a
-HIP event pair around every push launch.  A warm re-solve of unchanged inputs
-and a step that also copies the inputs in from caller buffers are reported
-beside it.
+HIP event pair around every push launch.  Re-solves of unchanged inputs (the
+first of them writes the input mask words, later ones copy them) and a step
+that also copies the inputs in from caller buffers are reported beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -491,7 +491,8 @@ def main():
     push_ms = sum(s.ms_push for s in ev_stats)
     push_launches = sum(s.push_launches for s in ev_stats)
     achieved = push_bytes / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0
-    # warm re-solve: unchanged inputs, masks kept from the previous solve
+    # re-solves of unchanged inputs: the first writes mask0 (a fresh solve
+    # does not), the others copy it
     warm_ms, wst = timed_steps(g, args.steps, fresh=False, **topt)
     assert all(s.inputs_fresh == 0 for s in wst)
     # inputs copied in from caller device buffers (validation included)
