@@ -472,3 +472,29 @@ def test_tile_order_gpu(gpu, method):
         assert np.all(np.diff(dist[order]) >= 0)
         assert mx == int(dist.max())  # every instance has a seed tile
         g.close()
+
+
+def test_mask_words_across_solves_gpu(gpu):
+    """The input mask words (mask0) are written by the first re-solve of
+    unchanged inputs, not by a fresh solve (except Fast-Cut's, which reads
+    them at once): fresh and repeated solves, alternating the two methods and
+    re-marking the inputs written, all against the oracle."""
+    rng = np.random.default_rng(31)
+    nx, ny = 150, 110
+    cs, ct, caps = random_grid(rng, nx, ny, 1, 4)
+    f, l, ok = ol.maxflow(nx, ny, 1, 4, cs, ct, caps)
+    g = gpu.GridGraph([(nx, ny, 1, 4)])
+    g.set_terminal_caps(cs, ct)
+    for d in range(4):
+        g.set_neighbor_caps(d, caps[d])
+    seq = [(False, {}), (False, dict(method=1)), (False, {}), (True, {}), (False, {}),
+           (True, dict(method=1)), (False, {}), (False, dict(method=1)), (True, {}), (True, {})]
+    for k, (fresh, opts) in enumerate(seq):
+        if fresh:
+            g.inputs_written()
+        say("case mask words", k, fresh, opts)
+        st = g.solve(**opts)
+        assert st.converged == 1
+        assert g.flow() == f, (k, opts)
+        assert np.array_equal(g.labels(), l), (k, opts)
+    g.close()
